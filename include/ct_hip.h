@@ -1,0 +1,153 @@
+/*
+ * ct_hip.h — C ABI of the MI355X FBP backend (libct_hip.so).
+ *
+ * Plain pointers and sizes only. Every entry point returns 0 on success or a
+ * negative ct_status; no C++ exception crosses this boundary. Host buffers
+ * are owned by the caller; device buffers made by ct_hip_alloc are owned by
+ * the context and freed by ct_hip_free / ct_hip_destroy. A context is
+ * internally synchronised (one recursive lock), so one context may be shared
+ * by threads, as the reference's controller shares one backend between its
+ * worker thread and an OpenMP loop (CtReconstructionController.cpp:1232-1241).
+ *
+ * Numerics follow the reference's SerialBackend (the method's oracle):
+ * geometry, normalisation and index rules of SerialBackend.cpp:59-318.
+ *
+ * Reference interfaces replaced (paths relative to QtCTReconstruction/):
+ *   IReconstructionBackend.h:10-55   the backend virtuals
+ *   IVolumeGenerator.h:16-27         the phantom generator
+ *   FilterDesign.h:9-18              filter design
+ *   IFFTBackend.h:16-42              the row filter's FFT
+ */
+#ifndef CT_HIP_H
+#define CT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ct_hip_ctx ct_hip_ctx;
+
+typedef enum {
+    CT_OK = 0,
+    CT_ERR_INVALID = -1,     /* bad argument (sizes, null pointers, filter id) */
+    CT_ERR_HIP = -2,         /* a HIP runtime call failed */
+    CT_ERR_NOMEM = -3,       /* device allocation failed */
+    CT_ERR_BUDGET = -4,      /* request exceeds the context's HBM budget */
+    CT_ERR_UNSUPPORTED = -5, /* shape outside what the kernels support */
+    CT_ERR_NODEVICE = -6     /* no HIP device */
+} ct_status;
+
+/* Filter ids: the order of ReconstructionParams::FilterType (CTTypes.h:151).
+ * Ids outside 0..5 select Shepp-Logan, as FilterDesign::createFilter's
+ * default branch does (FilterDesign.cpp:25-27). */
+enum { CT_FILTER_RAMP = 0, CT_FILTER_SHEPP_LOGAN = 1, CT_FILTER_HAMMING = 2,
+       CT_FILTER_COSINE = 3, CT_FILTER_HANN = 4, CT_FILTER_BARTLETT = 5 };
+
+/* ------------------------------------------------------------ context */
+
+/* Replaces IReconstructionBackend::isAvailable (IReconstructionBackend.h:18):
+ * number of visible HIP devices (0 when none). */
+int ct_hip_device_count(void);
+
+/* Creates a context on `device`. hbm_budget_bytes = 0 means "no limit
+ * beyond the device's free memory". Device state (trig tables, filter banks,
+ * workspaces) is created lazily and cached, like CUDABackend's mutable caches
+ * (CUDABackend.h:46-127). */
+int ct_hip_create(int device, size_t hbm_budget_bytes, ct_hip_ctx** out);
+void ct_hip_destroy(ct_hip_ctx* ctx);
+/* Message of the last failure on this context ("" if none). */
+const char* ct_hip_last_error(const ct_hip_ctx* ctx);
+/* Bytes of device memory the context holds now. */
+size_t ct_hip_bytes_in_use(const ct_hip_ctx* ctx);
+/* Frees cached workspaces (CUDABackend::clearWorkspace, CUDABackend.h:43). */
+int ct_hip_clear_workspace(ct_hip_ctx* ctx);
+
+/* ------------------------------------------------- device-resident buffers */
+int ct_hip_alloc(ct_hip_ctx* ctx, size_t bytes, void** dptr);
+int ct_hip_free(ct_hip_ctx* ctx, void* dptr);
+int ct_hip_upload(ct_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ct_hip_download(ct_hip_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ct_hip_synchronize(ct_hip_ctx* ctx);
+
+/* --------------------------------------------------------------- phantom */
+
+/* One ellipsoid of the project phantom: centre and semi-axes in the
+ * [-1,1]^3 voxel-centre coordinates (linspace(-1,1,n) per axis, as
+ * utils::linspace, Utils.cpp:12-26), ZYZ Euler angles, additive density. */
+typedef struct {
+    float cx, cy, cz;
+    float ax, ay, az;
+    float phi, theta, psi;
+    float rho;
+} ct_ellipsoid;
+
+/* Seeded parameters: centres U[-0.6,0.6]^3, semi-axes U[0.05,0.4]^3, angles
+ * U[0,pi)^3, density U[-0.3,1.0], from a counter-based generator so that
+ * every (seed, index) pair is reproducible on any host. Host-only. */
+int ct_phantom_params(uint64_t seed, int n, ct_ellipsoid* out);
+
+/* Replaces IVolumeGenerator::generateBrainHU (IVolumeGenerator.h:26;
+ * Generator3DGPU.cu:124-177): rasterises n ellipsoids into a (nz,ny,nx)
+ * float volume. _dev writes device memory (d_out), the other host memory. */
+int ct_hip_phantom(ct_hip_ctx* ctx, const ct_ellipsoid* e, int n, int nx, int ny, int nz,
+                   float* host_out);
+int ct_hip_phantom_dev(ct_hip_ctx* ctx, const ct_ellipsoid* e, int n, int nx, int ny, int nz,
+                       float* d_out);
+
+/* ------------------------------------------------------- backend virtuals */
+
+/* Replaces IReconstructionBackend::computeSinogram (IReconstructionBackend.h:21;
+ * SerialBackend.cpp:59-136). slice (h,w) row-major; sino_out [na][bins];
+ * minmax_out = {original_min_hu, original_max_hu}. Angles are 180/na*a deg. */
+int ct_hip_compute_sinogram(ct_hip_ctx* ctx, const float* slice, int w, int h, int num_angles,
+                            int bins, float* sino_out, float* minmax_out);
+
+/* Replaces IReconstructionBackend::reconstructSlice (IReconstructionBackend.h:24;
+ * SerialBackend.cpp:142-255): pad to ceil(sqrt2*bins), filter, backproject
+ * onto output_size^2, scale pi/(2na). Output is not denormalised, as Serial's. */
+int ct_hip_reconstruct_slice(ct_hip_ctx* ctx, const float* sino, int num_angles, int bins,
+                             const float* angles_deg, int output_size, int filter, float* out);
+
+/* Replaces IReconstructionBackend::reconstructVolume (IReconstructionBackend.h:28-31;
+ * SerialBackend.cpp:261-318): per z slice, sinogram with bins = w, FBP onto
+ * w x w, then v*eff_span + min. vol and out are (d,h,w); when h != w the
+ * output stays zero, as Buffer3D::setSlice drops mismatched slices
+ * (CTTypes.h:112-116). sino_out (optional) receives [d][na][w]. */
+int ct_hip_reconstruct_volume(ct_hip_ctx* ctx, const float* vol, int w, int h, int d,
+                              int num_angles, int filter, float* out, float* sino_out);
+
+/* Device-resident form of ct_hip_reconstruct_volume: all pointers are device
+ * memory; d_sino_out may be NULL. Enqueued on the context's stream and
+ * waited for before returning. */
+int ct_hip_reconstruct_volume_dev(ct_hip_ctx* ctx, const float* d_vol, int w, int h, int d,
+                                  int num_angles, int filter, float* d_out, float* d_sino_out);
+
+/* Replaces IReconstructionBackend::reconstructVolumeFromSinograms
+ * (IReconstructionBackend.h:36-41; default body BackendFactory.cpp:17-60):
+ * sinos (nz,na,bins) -> out (nz,bins,bins), each slice as reconstructSlice.
+ * Like the default body with a Serial backend, no denormalisation. */
+int ct_hip_reconstruct_from_sinograms(ct_hip_ctx* ctx, const float* sinos, int nz, int na, int bins,
+                                      const float* angles_deg, int filter, float* out);
+int ct_hip_reconstruct_from_sinograms_dev(ct_hip_ctx* ctx, const float* d_sinos, int nz, int na,
+                                          int bins, const float* angles_deg, int filter,
+                                          float* d_out);
+
+/* Replaces FilterDesign::createFilter (FilterDesign.h:11; FilterDesign.cpp:13-111),
+ * designed in FP64 on the device: n/2+1 taps, DC and Nyquist zeroed (the
+ * effective Serial filter: FFT::inverseReal never reads the Nyquist bin). */
+int ct_hip_design_filter(ct_hip_ctx* ctx, int n, int filter, float* out);
+
+/* Timings of the last volume call, milliseconds, measured with HIP events on
+ * the context's stream: [0] normalise+Radon, [1] row filter,
+ * [2] backprojection (incl. denormalisation), [3] total.
+ * Replaces lastSinogramTimeMs / lastReconstructionTimeMs
+ * (IReconstructionBackend.h:45-54). */
+int ct_hip_last_timings(const ct_hip_ctx* ctx, double* ms4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CT_HIP_H */

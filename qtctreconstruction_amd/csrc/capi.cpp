@@ -1,0 +1,622 @@
+// capi.cpp — the C ABI of libct_hip.so (declared in include/ct_hip.h):
+// context, HBM-budgeted device allocations, cached device state (trig tables,
+// filter banks, twiddles, workspaces) and the stage orchestration of the
+// reference's backend calls (IReconstructionBackend.h:10-55).
+#include "../../include/ct_hip.h"
+#include "ct_internal.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+using namespace cthip;
+
+namespace {
+
+struct Buf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct StatusError : std::runtime_error {
+    int code;
+    StatusError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+}  // namespace
+
+struct ct_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;  // 0 = unlimited
+    size_t in_use = 0;
+    std::recursive_mutex mu;
+    std::string err;
+    std::unordered_map<void*, size_t> sizes;  // every live device allocation
+    // workspaces (grow-only, reused across calls)
+    Buf vol, row, col, sino, filt, out, keys, mm, ell;
+    // cached device state
+    std::map<std::pair<int, int>, float*> filters;  // (nfft, type) -> H[nfft/2+1]
+    std::map<int, float2*> twiddles;                // nfft -> tw[nfft/2]
+    std::vector<uint32_t> trig_key;                 // bit pattern of the angle list
+    Buf trig_cos, trig_sin;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double timings[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess)
+        throw StatusError(CT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+void* dmalloc(ct_hip_ctx* c, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (c->budget && c->in_use + bytes > c->budget)
+        throw StatusError(CT_ERR_BUDGET, "HBM budget exceeded: need " + std::to_string(bytes) +
+                                             " B, in use " + std::to_string(c->in_use) +
+                                             " B of " + std::to_string(c->budget) + " B");
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        throw StatusError(CT_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + ") failed");
+    }
+    c->sizes[p] = bytes;
+    c->in_use += bytes;
+    return p;
+}
+
+void dfree(ct_hip_ctx* c, void* p) {
+    if (!p) return;
+    auto it = c->sizes.find(p);
+    if (it != c->sizes.end()) {
+        c->in_use -= it->second;
+        c->sizes.erase(it);
+    }
+    (void)hipFree(p);
+}
+
+template <typename T>
+T* ensure(ct_hip_ctx* c, Buf& b, size_t bytes) {
+    if (b.cap < bytes) {
+        if (b.p) {
+            hip_check(hipStreamSynchronize(c->stream), "sync before realloc");
+            dfree(c, b.p);
+            b.p = nullptr;
+            b.cap = 0;
+        }
+        b.p = dmalloc(c, bytes);
+        b.cap = bytes;
+    }
+    return static_cast<T*>(b.p);
+}
+
+void release(ct_hip_ctx* c, Buf& b) {
+    dfree(c, b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+// Serial's trig: th = degToRad(angle) in float, cos/sin by the C library's
+// float functions (SerialBackend.cpp:92-95, 198-204; Utils.cpp:44-46). The
+// table is built on the host because that is where those functions live; it
+// then stays resident and is read through the scalar cache.
+float deg2rad(float d) { return d * 3.14159265358979323846f / 180.0f; }
+
+void set_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
+    std::vector<uint32_t> key(na);
+    std::memcpy(key.data(), angles_deg, sizeof(float) * na);
+    if (key == c->trig_key && c->trig_cos.p) return;
+    std::vector<float> cs(na), sn(na);
+    for (int a = 0; a < na; ++a) {
+        const float th = deg2rad(angles_deg[a]);
+        cs[a] = std::cos(th);
+        sn[a] = std::sin(th);
+    }
+    float* dc = ensure<float>(c, c->trig_cos, sizeof(float) * na);
+    float* ds = ensure<float>(c, c->trig_sin, sizeof(float) * na);
+    hip_check(hipMemcpyAsync(dc, cs.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
+              "upload cos table");
+    hip_check(hipMemcpyAsync(ds, sn.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
+              "upload sin table");
+    hip_check(hipStreamSynchronize(c->stream), "trig upload");
+    c->trig_key = std::move(key);
+}
+
+std::vector<float> serial_angles(int na) {
+    std::vector<float> a(na);
+    const float step = 180.0f / (float)na;  // SerialBackend.cpp:92-93, 127-130
+    for (int i = 0; i < na; ++i) a[i] = step * (float)i;
+    return a;
+}
+
+int norm_filter(int f) { return (f >= 0 && f <= 5) ? f : CT_FILTER_SHEPP_LOGAN; }
+
+float* get_filter(ct_hip_ctx* c, int nfft, int type) {
+    type = norm_filter(type);
+    auto k = std::make_pair(nfft, type);
+    auto it = c->filters.find(k);
+    if (it != c->filters.end()) return it->second;
+    float* H = static_cast<float*>(dmalloc(c, sizeof(float) * (nfft / 2 + 1)));
+    hip_check(launch_filter_design(nfft, type, H, c->stream), "filter design");
+    c->filters[k] = H;
+    return H;
+}
+
+float2* get_twiddles(ct_hip_ctx* c, int nfft) {
+    auto it = c->twiddles.find(nfft);
+    if (it != c->twiddles.end()) return it->second;
+    float2* t = static_cast<float2*>(dmalloc(c, sizeof(float2) * (nfft / 2)));
+    hip_check(launch_twiddles(nfft, t, c->stream), "twiddles");
+    c->twiddles[nfft] = t;
+    return t;
+}
+
+struct Geom {
+    int bins, sq, pad, nfft;
+};
+
+// SerialBackend.cpp:152-161
+Geom geometry(int bins) {
+    Geom g;
+    g.bins = bins;
+    g.sq = (int)std::ceil(std::sqrt(2.0) * (double)bins);
+    g.pad = g.sq / 2 - bins / 2;
+    int p = 1;
+    while (p < 2 * g.sq) p <<= 1;
+    g.nfft = p > 64 ? p : 64;
+    return g;
+}
+
+void check_fft(const Geom& g) {
+    if (g.nfft > kMaxFFT)
+        throw StatusError(CT_ERR_UNSUPPORTED,
+                          "detector width " + std::to_string(g.bins) + " needs an FFT of " +
+                              std::to_string(g.nfft) + " > " + std::to_string(kMaxFFT));
+}
+
+size_t nzb_of(int nz) { return (size_t)((nz + kZB - 1) / kZB); }
+
+// normalise + Radon of nz slices (w x h) into d_sino [nz][na][bins]; d_mm gets {min,max} per slice
+void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na, int bins,
+                 float* d_sino, float* d_mm) {
+    unsigned* keys = ensure<unsigned>(c, c->keys, sizeof(unsigned) * 2 * (size_t)nz);
+    float4* row = ensure<float4>(c, c->row, sizeof(float4) * nzb_of(nz) * (size_t)w * h);
+    float4* col = ensure<float4>(c, c->col, sizeof(float4) * nzb_of(nz) * (size_t)w * h);
+    hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
+    hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
+    hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
+    hip_check(launch_normalise(d_vol, w, h, nz, d_mm, row, col, c->stream), "normalise");
+    hip_check(launch_radon(row, col, w, h, nz, na, bins, (const float*)c->trig_cos.p,
+                           (const float*)c->trig_sin.p, d_sino, c->stream),
+              "radon");
+}
+
+// filter + backprojection of nz sinograms [nz][na][bins] onto N x N
+void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int N, int filter,
+               const float* d_mm, float* d_out) {
+    const Geom g = geometry(bins);
+    check_fft(g);
+    const float* H = get_filter(c, g.nfft, filter);
+    const float2* tw = get_twiddles(c, g.nfft);
+    float4* filt = ensure<float4>(c, c->filt, sizeof(float4) * nzb_of(nz) * (size_t)na * g.sq);
+    hip_check(hipEventRecord(c->ev[1], c->stream), "event");
+    hip_check(launch_row_filter_fft(d_sino, nz, na, bins, g.sq, g.pad, g.nfft, H, tw, filt,
+                                    c->stream),
+              "row filter");
+    hip_check(hipEventRecord(c->ev[2], c->stream), "event");
+    const float scale = 3.14159265358979323846f / (2.0f * (float)na);  // SerialBackend.cpp:246
+    hip_check(launch_backproject(filt, nz, na, g.sq, N, (const float*)c->trig_cos.p,
+                                 (const float*)c->trig_sin.p, scale, d_mm, d_out, c->stream),
+              "backprojection");
+    hip_check(hipEventRecord(c->ev[3], c->stream), "event");
+}
+
+void read_timings(ct_hip_ctx* c, bool with_radon) {
+    hip_check(hipEventSynchronize(c->ev[3]), "event sync");
+    float t01 = 0, t12 = 0, t23 = 0;
+    if (with_radon) hip_check(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]), "elapsed");
+    hip_check(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]), "elapsed");
+    hip_check(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]), "elapsed");
+    c->timings[0] = t01;
+    c->timings[1] = t12;
+    c->timings[2] = t23;
+    c->timings[3] = (double)t01 + t12 + t23;
+}
+
+void bind_device(ct_hip_ctx* c) { hip_check(hipSetDevice(c->device), "hipSetDevice"); }
+
+template <typename F>
+int guarded(ct_hip_ctx* c, F&& f) {
+    if (!c) return CT_ERR_INVALID;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    try {
+        bind_device(c);
+        f();
+        c->err.clear();
+        return CT_OK;
+    } catch (const StatusError& e) {
+        c->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        c->err = "host allocation failed";
+        return CT_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return CT_ERR_INVALID;
+    }
+}
+
+void need(bool ok, const char* msg) {
+    if (!ok) throw StatusError(CT_ERR_INVALID, msg);
+}
+
+// ---- phantom parameters (project spec) -----------------------------------
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+float uniform_draw(uint64_t seed, int e, int p, double lo, double hi) {
+    const uint64_t x = splitmix64(seed ^ splitmix64((uint64_t)e * 16u + (uint64_t)p));
+    const double u = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+    return (float)(lo + (hi - lo) * u);
+}
+
+PhEll derive(const ct_ellipsoid& e) {
+    const double a = e.phi, b = e.theta, g = e.psi;
+    const double ca = std::cos(a), sa = std::sin(a), cb = std::cos(b), sb = std::sin(b),
+                 cg = std::cos(g), sg = std::sin(g);
+    const double Rz1[3][3] = {{ca, -sa, 0}, {sa, ca, 0}, {0, 0, 1}};
+    const double Ry[3][3] = {{cb, 0, sb}, {0, 1, 0}, {-sb, 0, cb}};
+    const double Rz2[3][3] = {{cg, -sg, 0}, {sg, cg, 0}, {0, 0, 1}};
+    double T[3][3], R[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += Rz1[i][k] * Ry[k][j];
+            T[i][j] = s;
+        }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += T[i][k] * Rz2[k][j];
+            R[i][j] = s;
+        }
+    PhEll q;
+    q.cx = e.cx;
+    q.cy = e.cy;
+    q.cz = e.cz;
+    // body coordinates q = R^T (p - c)
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) q.m[i * 3 + j] = (float)R[j][i];
+    const double ax[3] = {e.ax, e.ay, e.az};
+    for (int k = 0; k < 3; ++k) q.ia[k] = (float)(1.0 / (ax[k] * ax[k]));
+    q.rho = e.rho;
+    const double cc[3] = {e.cx, e.cy, e.cz};
+    for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += (R[j][k] * ax[k]) * (R[j][k] * ax[k]);
+        const double ext = std::sqrt(s) + 1e-3;
+        q.lo[j] = (float)(cc[j] - ext);
+        q.hi[j] = (float)(cc[j] + ext);
+    }
+    return q;
+}
+
+void phantom_to_device(ct_hip_ctx* c, const ct_ellipsoid* e, int n, int nx, int ny, int nz,
+                       float* d_out) {
+    need(nx > 0 && ny > 0 && nz > 0 && n >= 0 && (n == 0 || e), "phantom: bad arguments");
+    std::vector<PhEll> h(n > 0 ? n : 1);
+    for (int k = 0; k < n; ++k) h[k] = derive(e[k]);
+    PhEll* d = ensure<PhEll>(c, c->ell, sizeof(PhEll) * h.size());
+    hip_check(hipMemcpyAsync(d, h.data(), sizeof(PhEll) * h.size(), hipMemcpyHostToDevice,
+                             c->stream),
+              "upload ellipsoids");
+    hip_check(launch_phantom(d, n, nx, ny, nz, d_out, c->stream), "phantom");
+    hip_check(hipStreamSynchronize(c->stream), "phantom sync");
+}
+
+}  // namespace
+
+// =========================================================================
+extern "C" {
+
+int ct_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int ct_hip_create(int device, size_t hbm_budget_bytes, ct_hip_ctx** out) {
+    if (!out) return CT_ERR_INVALID;
+    *out = nullptr;
+    const int n = ct_hip_device_count();
+    if (n <= 0) return CT_ERR_NODEVICE;
+    if (device < 0 || device >= n) return CT_ERR_INVALID;
+    ct_hip_ctx* c = new (std::nothrow) ct_hip_ctx();
+    if (!c) return CT_ERR_NOMEM;
+    c->device = device;
+    c->budget = hbm_budget_bytes;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return CT_ERR_HIP;
+    }
+    for (auto& e : c->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            ct_hip_destroy(c);
+            return CT_ERR_HIP;
+        }
+    }
+    *out = c;
+    return CT_OK;
+}
+
+void ct_hip_destroy(ct_hip_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->sizes) (void)hipFree(kv.first);
+    c->sizes.clear();
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* ct_hip_last_error(const ct_hip_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+size_t ct_hip_bytes_in_use(const ct_hip_ctx* c) { return c ? c->in_use : 0; }
+
+int ct_hip_clear_workspace(ct_hip_ctx* c) {
+    return guarded(c, [&] {
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        for (Buf* b : {&c->vol, &c->row, &c->col, &c->sino, &c->filt, &c->out, &c->keys, &c->mm,
+                       &c->ell, &c->trig_cos, &c->trig_sin})
+            release(c, *b);
+        c->trig_key.clear();
+        for (auto& kv : c->filters) dfree(c, kv.second);
+        for (auto& kv : c->twiddles) dfree(c, kv.second);
+        c->filters.clear();
+        c->twiddles.clear();
+    });
+}
+
+int ct_hip_alloc(ct_hip_ctx* c, size_t bytes, void** dptr) {
+    return guarded(c, [&] {
+        need(dptr != nullptr, "alloc: null out pointer");
+        *dptr = dmalloc(c, bytes);
+    });
+}
+
+int ct_hip_free(ct_hip_ctx* c, void* p) {
+    return guarded(c, [&] {
+        need(c->sizes.count(p) > 0, "free: pointer not owned by this context");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        dfree(c, p);
+    });
+}
+
+int ct_hip_upload(ct_hip_ctx* c, void* dst, const void* src, size_t bytes) {
+    return guarded(c, [&] {
+        need(dst && src, "upload: null pointer");
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
+        hip_check(hipStreamSynchronize(c->stream), "upload sync");
+    });
+}
+
+int ct_hip_download(ct_hip_ctx* c, void* dst, const void* src, size_t bytes) {
+    return guarded(c, [&] {
+        need(dst && src, "download: null pointer");
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "download");
+        hip_check(hipStreamSynchronize(c->stream), "download sync");
+    });
+}
+
+int ct_hip_synchronize(ct_hip_ctx* c) {
+    return guarded(c, [&] { hip_check(hipStreamSynchronize(c->stream), "sync"); });
+}
+
+int ct_phantom_params(uint64_t seed, int n, ct_ellipsoid* out) {
+    if (n < 0 || (n > 0 && !out)) return CT_ERR_INVALID;
+    const double pi = 3.14159265358979323846;
+    for (int e = 0; e < n; ++e) {
+        ct_ellipsoid& q = out[e];
+        q.cx = uniform_draw(seed, e, 0, -0.6, 0.6);
+        q.cy = uniform_draw(seed, e, 1, -0.6, 0.6);
+        q.cz = uniform_draw(seed, e, 2, -0.6, 0.6);
+        q.ax = uniform_draw(seed, e, 3, 0.05, 0.4);
+        q.ay = uniform_draw(seed, e, 4, 0.05, 0.4);
+        q.az = uniform_draw(seed, e, 5, 0.05, 0.4);
+        q.phi = uniform_draw(seed, e, 6, 0.0, pi);
+        q.theta = uniform_draw(seed, e, 7, 0.0, pi);
+        q.psi = uniform_draw(seed, e, 8, 0.0, pi);
+        q.rho = uniform_draw(seed, e, 9, -0.3, 1.0);
+    }
+    return CT_OK;
+}
+
+int ct_hip_phantom_dev(ct_hip_ctx* c, const ct_ellipsoid* e, int n, int nx, int ny, int nz,
+                       float* d_out) {
+    return guarded(c, [&] {
+        need(d_out != nullptr, "phantom: null output");
+        phantom_to_device(c, e, n, nx, ny, nz, d_out);
+    });
+}
+
+int ct_hip_phantom(ct_hip_ctx* c, const ct_ellipsoid* e, int n, int nx, int ny, int nz,
+                   float* host_out) {
+    return guarded(c, [&] {
+        need(host_out != nullptr, "phantom: null output");
+        need(nx > 0 && ny > 0 && nz > 0, "phantom: bad shape");
+        const size_t bytes = sizeof(float) * (size_t)nx * ny * nz;
+        float* d = ensure<float>(c, c->vol, bytes);
+        phantom_to_device(c, e, n, nx, ny, nz, d);
+        hip_check(hipMemcpy(host_out, d, bytes, hipMemcpyDeviceToHost), "download phantom");
+    });
+}
+
+int ct_hip_compute_sinogram(ct_hip_ctx* c, const float* slice, int w, int h, int na, int bins,
+                            float* sino_out, float* minmax_out) {
+    return guarded(c, [&] {
+        need(slice && sino_out, "computeSinogram: null pointer");
+        need(w > 0 && h > 0 && na > 0 && bins > 0, "computeSinogram: empty input");
+        const size_t pix = (size_t)w * h;
+        float* d_in = ensure<float>(c, c->vol, sizeof(float) * pix);
+        float* d_sino = ensure<float>(c, c->sino, sizeof(float) * (size_t)na * bins);
+        float* d_mm = ensure<float>(c, c->mm, sizeof(float) * 2);
+        const std::vector<float> ang = serial_angles(na);
+        set_trig(c, ang.data(), na);
+        hip_check(hipMemcpyAsync(d_in, slice, sizeof(float) * pix, hipMemcpyHostToDevice, c->stream),
+                  "upload slice");
+        stage_radon(c, d_in, w, h, 1, na, bins, d_sino, d_mm);
+        hip_check(hipMemcpyAsync(sino_out, d_sino, sizeof(float) * (size_t)na * bins,
+                                 hipMemcpyDeviceToHost, c->stream),
+                  "download sinogram");
+        float mm[2];
+        hip_check(hipMemcpyAsync(mm, d_mm, sizeof(mm), hipMemcpyDeviceToHost, c->stream), "mm");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        if (minmax_out) {
+            minmax_out[0] = mm[0];
+            minmax_out[1] = mm[1];
+        }
+    });
+}
+
+int ct_hip_reconstruct_slice(ct_hip_ctx* c, const float* sino, int na, int bins,
+                             const float* angles_deg, int N, int filter, float* out) {
+    return guarded(c, [&] {
+        need(sino && angles_deg && out, "reconstructSlice: null pointer");
+        need(na > 0 && bins > 0 && N > 0, "reconstructSlice: empty input");
+        const size_t n_in = (size_t)na * bins;
+        float* d_sino = ensure<float>(c, c->sino, sizeof(float) * n_in);
+        float* d_out = ensure<float>(c, c->out, sizeof(float) * (size_t)N * N);
+        set_trig(c, angles_deg, na);
+        hip_check(hipMemcpyAsync(d_sino, sino, sizeof(float) * n_in, hipMemcpyHostToDevice,
+                                 c->stream),
+                  "upload sinogram");
+        stage_fbp(c, d_sino, 1, na, bins, N, filter, nullptr, d_out);
+        hip_check(hipMemcpyAsync(out, d_out, sizeof(float) * (size_t)N * N, hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "download slice");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+static void volume_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, int na, int filter,
+                       float* d_out, float* d_sino_out) {
+    need(d_vol && d_out, "reconstructVolume: null pointer");
+    need(w > 0 && h > 0 && d > 0 && na > 0, "reconstructVolume: empty input");
+    const int bins = w;  // SerialBackend.cpp:287-288
+    const std::vector<float> ang = serial_angles(na);
+    set_trig(c, ang.data(), na);
+    float* d_sino = d_sino_out ? d_sino_out
+                               : ensure<float>(c, c->sino, sizeof(float) * (size_t)d * na * bins);
+    float* d_mm = ensure<float>(c, c->mm, sizeof(float) * 2 * (size_t)d);
+    hip_check(hipEventRecord(c->ev[0], c->stream), "event");
+    stage_radon(c, d_vol, w, h, d, na, bins, d_sino, d_mm);
+    if (h == w) {
+        stage_fbp(c, d_sino, d, na, bins, w, filter, d_mm, d_out);
+        read_timings(c, true);
+    } else {
+        // Buffer3D::setSlice drops slices whose shape differs (CTTypes.h:112-116)
+        hip_check(hipMemsetAsync(d_out, 0, sizeof(float) * (size_t)w * h * d, c->stream), "zero");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    }
+}
+
+int ct_hip_reconstruct_volume_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, int na,
+                                  int filter, float* d_out, float* d_sino_out) {
+    return guarded(c, [&] {
+        volume_dev(c, d_vol, w, h, d, na, filter, d_out, d_sino_out);
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int ct_hip_reconstruct_volume(ct_hip_ctx* c, const float* vol, int w, int h, int d, int na,
+                              int filter, float* out, float* sino_out) {
+    return guarded(c, [&] {
+        need(vol && out, "reconstructVolume: null pointer");
+        need(w > 0 && h > 0 && d > 0 && na > 0, "reconstructVolume: empty input");
+        const size_t n = (size_t)w * h * d;
+        float* d_vol = ensure<float>(c, c->vol, sizeof(float) * n);
+        float* d_out = ensure<float>(c, c->out, sizeof(float) * n);
+        hip_check(hipMemcpyAsync(d_vol, vol, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
+                  "upload volume");
+        volume_dev(c, d_vol, w, h, d, na, filter, d_out, nullptr);
+        hip_check(hipMemcpyAsync(out, d_out, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream),
+                  "download volume");
+        if (sino_out)
+            hip_check(hipMemcpyAsync(sino_out, c->sino.p, sizeof(float) * (size_t)d * na * w,
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "download sinograms");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+static void from_sinos_dev(ct_hip_ctx* c, const float* d_sinos, int nz, int na, int bins,
+                           const float* angles_deg, int filter, float* d_out) {
+    need(d_sinos && angles_deg && d_out, "reconstructVolumeFromSinograms: null pointer");
+    need(nz > 0 && na > 0 && bins > 0, "reconstructVolumeFromSinograms: empty input");
+    set_trig(c, angles_deg, na);
+    stage_fbp(c, d_sinos, nz, na, bins, bins, filter, nullptr, d_out);
+    read_timings(c, false);
+}
+
+int ct_hip_reconstruct_from_sinograms_dev(ct_hip_ctx* c, const float* d_sinos, int nz, int na,
+                                          int bins, const float* angles_deg, int filter,
+                                          float* d_out) {
+    return guarded(c, [&] { from_sinos_dev(c, d_sinos, nz, na, bins, angles_deg, filter, d_out); });
+}
+
+int ct_hip_reconstruct_from_sinograms(ct_hip_ctx* c, const float* sinos, int nz, int na, int bins,
+                                      const float* angles_deg, int filter, float* out) {
+    return guarded(c, [&] {
+        need(sinos && out, "reconstructVolumeFromSinograms: null pointer");
+        need(nz > 0 && na > 0 && bins > 0, "reconstructVolumeFromSinograms: empty input");
+        const size_t nin = (size_t)nz * na * bins, nout = (size_t)nz * bins * bins;
+        float* d_in = ensure<float>(c, c->sino, sizeof(float) * nin);
+        float* d_out = ensure<float>(c, c->out, sizeof(float) * nout);
+        hip_check(hipMemcpyAsync(d_in, sinos, sizeof(float) * nin, hipMemcpyHostToDevice, c->stream),
+                  "upload sinograms");
+        from_sinos_dev(c, d_in, nz, na, bins, angles_deg, filter, d_out);
+        hip_check(hipMemcpyAsync(out, d_out, sizeof(float) * nout, hipMemcpyDeviceToHost, c->stream),
+                  "download volume");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int ct_hip_design_filter(ct_hip_ctx* c, int n, int filter, float* out) {
+    return guarded(c, [&] {
+        need(out != nullptr, "designFilter: null output");
+        need(n >= 4 && (n & (n - 1)) == 0, "designFilter: n must be a power of two >= 4");
+        const float* H = get_filter(c, n, filter);
+        hip_check(hipMemcpyAsync(out, H, sizeof(float) * (n / 2 + 1), hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "download filter");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int ct_hip_last_timings(const ct_hip_ctx* c, double* ms4) {
+    if (!c || !ms4) return CT_ERR_INVALID;
+    for (int i = 0; i < 4; ++i) ms4[i] = c->timings[i];
+    return CT_OK;
+}
+
+}  // extern "C"

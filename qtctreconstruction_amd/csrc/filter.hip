@@ -1,0 +1,189 @@
+// filter.hip — row filter of the FBP: filter design (FP64) and the padded
+// FFT convolution of every sinogram row.
+//
+// Reference semantics (SerialBackend.cpp:150-195, FilterDesign.cpp:13-111,
+// FFT.cpp:51-120): each row of `bins` samples is placed at offset
+// pad = sq/2 - bins/2 in a zero row of nfft = max(64, nextPow2(2*sq)),
+// transformed, multiplied by the real filter H (n/2+1 taps, H[0] = 0),
+// transformed back, and the first sq samples are kept. The reference's
+// inverse never reads the Nyquist bin, so the effective filter also has
+// H[n/2] = 0; the design here zeroes it.
+//
+// Design: FP64 on the device, by direct summation of the DFT of the
+// reference's float kernel f (f[0]=0.25, f[odd m] = -1/(pi m)^2 rounded to
+// float), times the window, stored as float.
+//
+// Filtering: the filter is real and even, so two real rows ride in one
+// complex FFT (row A in the real part, row B in the imaginary part) and come
+// back separated. A block owns one angle of one z-block (4 rows -> 2 complex
+// FFTs) held in LDS: decimation-in-time forward from bit-reversed input,
+// multiply by H in natural order, decimation-in-frequency inverse whose
+// output is read back at bit-reversed positions. Twiddles live in LDS.
+// Output is written z-innermost (float4 = 4 slices per bin), the layout the
+// backprojection stages from.
+#include "ct_internal.h"
+
+namespace cthip {
+
+__global__ void filter_design_kernel(int n, int type, float* __restrict__ H) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nh = n / 2;
+    if (k > nh) return;
+    const double pi = 3.14159265358979323846;
+    double acc = 0.25;  // f[0] * cos(0)
+    for (int j = 1; j < n; j += 2) {
+        const int m = (j < nh) ? j : n - j;
+        const double t = 1.0 / (pi * (double)m);
+        const double fj = (double)(float)(-1.0 * t * t);
+        const long long r = ((long long)j * k) % n;
+        acc += fj * cos(2.0 * pi * (double)r / (double)n);
+    }
+    double h = 2.0 * acc;
+    const double freq = (double)k / (double)n;
+    switch (type) {
+        case 0: break;                                                  // Ramp
+        case 2: h *= 0.54 + 0.46 * cos(pi * freq / 0.5); break;        // Hamming
+        case 3: h *= cos(pi * freq); break;                             // Cosine
+        case 4: h *= 0.5 + 0.5 * cos(pi * freq / 0.5); break;          // Hann
+        case 5: {                                                        // Bartlett
+            const double r = fabs(freq) / 0.5;
+            h *= 1.0 - (r < 1.0 ? r : 1.0);
+        } break;
+        case 1:
+        default: {  // Shepp-Logan (also every unknown id)
+            const double om = pi * freq;
+            if (k > 0) h *= sin(om) / om;
+        } break;
+    }
+    if (k == 0 || k == nh) h = 0.0;
+    H[k] = (float)h;
+}
+
+__global__ void twiddle_kernel(int n, float2* __restrict__ tw) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n / 2) return;
+    const double a = -2.0 * 3.14159265358979323846 * (double)k / (double)n;
+    tw[k] = make_float2((float)cos(a), (float)sin(a));
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+__global__ __launch_bounds__(256) void row_filter_fft_kernel(
+    const float* __restrict__ sino, int nz, int na, int bins, int sq, int pad, int n, int logn,
+    const float* __restrict__ Hh, const float2* __restrict__ tw, float4* __restrict__ filt) {
+    extern __shared__ float2 sm[];
+    float2* X0 = sm;
+    float2* X1 = sm + n;
+    float2* T = sm + 2 * n;
+    const int a = blockIdx.x;
+    const int zb = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int nh = n / 2;
+    for (int k = tid; k < nh; k += 256) T[k] = tw[k];
+    const float* rows[kZB];
+#pragma unroll
+    for (int r = 0; r < kZB; ++r) {
+        const int z = zb * kZB + r;
+        rows[r] = (z < nz) ? sino + ((size_t)z * na + a) * bins : nullptr;
+    }
+    for (int j = tid; j < n; j += 256) {
+        const int jj = j - pad;
+        float v[kZB];
+#pragma unroll
+        for (int r = 0; r < kZB; ++r) v[r] = (rows[r] && jj >= 0 && jj < bins) ? rows[r][jj] : 0.0f;
+        const int br = (int)(__brev((unsigned)j) >> (32 - logn));
+        X0[br] = make_float2(v[0], v[1]);
+        X1[br] = make_float2(v[2], v[3]);
+    }
+    __syncthreads();
+    // forward: decimation in time, natural-order output
+    for (int len = 2; len <= n; len <<= 1) {
+        const int half = len >> 1;
+        const int tstep = n / len;
+        for (int b = tid; b < nh; b += 256) {
+            const int g = b / half, k = b - g * half;
+            const int i0 = g * len + k, i1 = i0 + half;
+            const float2 w = T[k * tstep];
+            const float2 u0 = X0[i0], v0 = cmul(X0[i1], w);
+            const float2 u1 = X1[i0], v1 = cmul(X1[i1], w);
+            X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
+            X0[i1] = make_float2(u0.x - v0.x, u0.y - v0.y);
+            X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
+            X1[i1] = make_float2(u1.x - v1.x, u1.y - v1.y);
+        }
+        __syncthreads();
+    }
+    // multiply by the real, even filter (H[n-k] = H[k])
+    for (int k = tid; k < n; k += 256) {
+        const float hv = Hh[k <= nh ? k : n - k];
+        X0[k] = make_float2(X0[k].x * hv, X0[k].y * hv);
+        X1[k] = make_float2(X1[k].x * hv, X1[k].y * hv);
+    }
+    __syncthreads();
+    // inverse: decimation in frequency, bit-reversed output
+    for (int len = n; len >= 2; len >>= 1) {
+        const int half = len >> 1;
+        const int tstep = n / len;
+        for (int b = tid; b < nh; b += 256) {
+            const int g = b / half, k = b - g * half;
+            const int i0 = g * len + k, i1 = i0 + half;
+            const float2 w = T[k * tstep];
+            const float2 u0 = X0[i0], v0 = X0[i1];
+            const float2 u1 = X1[i0], v1 = X1[i1];
+            X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
+            X0[i1] = cmulc(make_float2(u0.x - v0.x, u0.y - v0.y), w);
+            X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
+            X1[i1] = cmulc(make_float2(u1.x - v1.x, u1.y - v1.y), w);
+        }
+        __syncthreads();
+    }
+    const float inv_n = 1.0f / (float)n;
+    float4* dst = filt + ((size_t)zb * na + a) * sq;
+    for (int i = tid; i < sq; i += 256) {
+        const int br = (int)(__brev((unsigned)i) >> (32 - logn));
+        const float2 p = X0[br], q = X1[br];
+        dst[i] = make_float4(p.x * inv_n, p.y * inv_n, q.x * inv_n, q.y * inv_n);
+    }
+}
+
+hipError_t launch_filter_design(int n, int type, float* d_H, hipStream_t s) {
+    const int cnt = n / 2 + 1;
+    hipLaunchKernelGGL(filter_design_kernel, dim3((cnt + 127) / 128), dim3(128), 0, s, n, type,
+                       d_H);
+    return hipGetLastError();
+}
+
+hipError_t launch_twiddles(int n, float2* d_tw, hipStream_t s) {
+    const int cnt = n / 2;
+    hipLaunchKernelGGL(twiddle_kernel, dim3((cnt + 127) / 128), dim3(128), 0, s, n, d_tw);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_filter_fft(const float* d_sino, int nz, int na, int bins, int sq, int pad,
+                                 int nfft, const float* d_H, const float2* d_tw, float4* d_filt,
+                                 hipStream_t s) {
+    if (nfft < 4 || nfft > kMaxFFT || (nfft & (nfft - 1))) return hipErrorInvalidValue;
+    int logn = 0;
+    while ((1 << logn) < nfft) ++logn;
+    const size_t lds = sizeof(float2) * (size_t)(2 * nfft + nfft / 2);
+    static bool attr_set = false;  // > 64 KB of dynamic LDS must be opted into
+    if (!attr_set) {
+        const hipError_t e = hipFuncSetAttribute((const void*)row_filter_fft_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)(sizeof(float2) * (2 * kMaxFFT + kMaxFFT / 2)));
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    const int nzb = (nz + kZB - 1) / kZB;
+    dim3 grid(na, nzb);
+    hipLaunchKernelGGL(row_filter_fft_kernel, grid, dim3(256), lds, s, d_sino, nz, na, bins, sq,
+                       pad, nfft, logn, d_H, d_tw, d_filt);
+    return hipGetLastError();
+}
+
+}  // namespace cthip
