@@ -135,10 +135,40 @@ int ct_hip_reconstruct_from_sinograms_dev(ct_hip_ctx* ctx, const float* d_sinos,
                                           int bins, const float* angles_deg, int filter,
                                           float* d_out);
 
+/* Replaces IReconstructionBackend::extractPointCloud (IReconstructionBackend.h:44;
+ * SerialBackend.cpp:325-346): every voxel with value > threshold, in z,y,x
+ * order, as 4 floats (x, y, z, hu). xc/yc/zc are the axis coordinates (host
+ * arrays of w, h, d floats) or NULL for the indices. *count always receives
+ * the number of points; points are written only when out_xyzh is not NULL
+ * and capacity >= *count. The _dev form takes the volume in device memory. */
+int ct_hip_extract_point_cloud(ct_hip_ctx* ctx, const float* vol, int w, int h, int d,
+                               const float* xc, const float* yc, const float* zc, float threshold,
+                               float* out_xyzh, size_t capacity, size_t* count);
+int ct_hip_extract_point_cloud_dev(ct_hip_ctx* ctx, const float* d_vol, int w, int h, int d,
+                                   const float* xc, const float* yc, const float* zc,
+                                   float threshold, float* out_xyzh, size_t capacity,
+                                   size_t* count);
+
 /* Replaces FilterDesign::createFilter (FilterDesign.h:11; FilterDesign.cpp:13-111),
  * designed in FP64 on the device: n/2+1 taps, DC and Nyquist zeroed (the
  * effective Serial filter: FFT::inverseReal never reads the Nyquist bin). */
 int ct_hip_design_filter(ct_hip_ctx* ctx, int n, int filter, float* out);
+
+/* Row-filter backend of the context, the analogue of the reference's
+ * compile-time FFT backend choice (IFFTBackend.h:16-42, CMake FFT_BACKEND):
+ *   CT_FILTER_BACKEND_FFT_F32        LDS FFT in FP32 (default)
+ *   CT_FILTER_BACKEND_GEMM_BF16      Toeplitz GEMM on MFMA, BF16 inputs, FP32 accumulate
+ *   CT_FILTER_BACKEND_GEMM_SPLIT_BF16 hi+lo BF16, three products, FP32 accumulate
+ *   CT_FILTER_BACKEND_GEMM_FP8       FP8 E4M3 inputs with per-row scale
+ * Backprojection always accumulates in FP32. */
+enum { CT_FILTER_BACKEND_FFT_F32 = 0, CT_FILTER_BACKEND_GEMM_BF16 = 1,
+       CT_FILTER_BACKEND_GEMM_SPLIT_BF16 = 2, CT_FILTER_BACKEND_GEMM_FP8 = 3 };
+int ct_hip_set_filter_backend(ct_hip_ctx* ctx, int backend);
+
+/* The row-filter step alone (SerialBackend.cpp:150-195): sinos (nz,na,bins) ->
+ * out (nz,na,sq), sq = ceil(sqrt2*bins), with the context's filter backend. */
+int ct_hip_filter_sinograms(ct_hip_ctx* ctx, const float* sinos, int nz, int na, int bins,
+                            int filter, float* out);
 
 /* Timings of the last volume call, milliseconds, measured with HIP events on
  * the context's stream: [0] normalise+Radon, [1] row filter,

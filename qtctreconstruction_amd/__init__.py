@@ -31,6 +31,14 @@ class FilterType(enum.IntEnum):
     Bartlett = 5
 
 
+class FilterBackend(enum.IntEnum):
+    """Row-filter backends (CT_FILTER_BACKEND_* in include/ct_hip.h)."""
+    FFT_F32 = 0
+    GEMM_BF16 = 1
+    GEMM_SPLIT_BF16 = 2
+    GEMM_FP8 = 3
+
+
 class BackendError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[ct_hip {code}] {msg}")
@@ -84,7 +92,13 @@ def load_library(path: str = LIB_PATH):
         "ct_hip_reconstruct_volume_dev": (i, [ctx, _vp, i, i, i, i, i, _vp, _vp]),
         "ct_hip_reconstruct_from_sinograms": (i, [ctx, _f32p, i, i, i, _f32p, i, _f32p]),
         "ct_hip_reconstruct_from_sinograms_dev": (i, [ctx, _vp, i, i, i, _f32p, i, _vp]),
+        "ct_hip_extract_point_cloud": (i, [ctx, _f32p, i, i, i, _f32p, _f32p, _f32p, ctypes.c_float,
+                                           _f32p, sz, ctypes.POINTER(sz)]),
+        "ct_hip_extract_point_cloud_dev": (i, [ctx, _vp, i, i, i, _f32p, _f32p, _f32p,
+                                               ctypes.c_float, _f32p, sz, ctypes.POINTER(sz)]),
         "ct_hip_design_filter": (i, [ctx, i, i, _f32p]),
+        "ct_hip_set_filter_backend": (i, [ctx, i]),
+        "ct_hip_filter_sinograms": (i, [ctx, _f32p, i, i, i, i, _f32p]),
         "ct_hip_last_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
@@ -104,7 +118,9 @@ def exported_symbols() -> List[str]:
         "ct_hip_phantom", "ct_hip_phantom_dev", "ct_hip_compute_sinogram",
         "ct_hip_reconstruct_slice", "ct_hip_reconstruct_volume", "ct_hip_reconstruct_volume_dev",
         "ct_hip_reconstruct_from_sinograms", "ct_hip_reconstruct_from_sinograms_dev",
-        "ct_hip_design_filter", "ct_hip_last_timings",
+        "ct_hip_extract_point_cloud", "ct_hip_extract_point_cloud_dev",
+        "ct_hip_design_filter", "ct_hip_set_filter_backend", "ct_hip_filter_sinograms",
+        "ct_hip_last_timings",
     ]
 
 
@@ -222,6 +238,18 @@ class Context:
         self._call("ct_hip_design_filter", self.handle, n, int(ftype), _p(out))
         return out
 
+    def set_filter_backend(self, backend: int):
+        self._call("ct_hip_set_filter_backend", self.handle, int(backend))
+
+    def filter_sinograms(self, sinos, ftype: int) -> np.ndarray:
+        """The row-filter step alone: (nz,na,bins) -> (nz,na,sq)."""
+        s = _f32(sinos)
+        nz, na, bins = s.shape
+        sq = int(np.ceil(np.sqrt(2.0) * bins))
+        out = np.empty((nz, na, sq), np.float32)
+        self._call("ct_hip_filter_sinograms", self.handle, _p(s), nz, na, bins, int(ftype), _p(out))
+        return out
+
     def phantom(self, seed: int, n: int, nx: int, ny: int, nz: int) -> np.ndarray:
         e = phantom_params(seed, n)
         out = np.empty((nz, ny, nx), np.float32)
@@ -310,6 +338,24 @@ class HipBackend:
         if onSliceDone:
             for z in range(nz):
                 onSliceDone(z, out[z])
+        return out
+
+    def extractPointCloud(self, vol, threshold: float, x_coords=None, y_coords=None,
+                          z_coords=None) -> np.ndarray:
+        """Points (x, y, z, hu) of voxels with hu > threshold, in z,y,x order."""
+        v = _f32(vol)
+        if v.size == 0:
+            return np.zeros((0, 4), np.float32)
+        d, h, w = v.shape
+        cs = [None if c is None else _f32(c) for c in (x_coords, y_coords, z_coords)]
+        cp = [None if c is None else _p(c) for c in cs]
+        n = ctypes.c_size_t(0)
+        self.ctx._call("ct_hip_extract_point_cloud", self.ctx.handle, _p(v), w, h, d, *cp,
+                       float(threshold), None, 0, ctypes.byref(n))
+        out = np.empty((n.value, 4), np.float32)
+        if n.value:
+            self.ctx._call("ct_hip_extract_point_cloud", self.ctx.handle, _p(v), w, h, d, *cp,
+                           float(threshold), _p(out), n.value, ctypes.byref(n))
         return out
 
     def lastSinogramTimeMs(self) -> float:

@@ -26,7 +26,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract
           "-Wall", "-Wno-unused-command-line-argument", "-Wno-unknown-pragmas",
           "-I", os.path.join(os.path.dirname(PKG), "include")]
 
-SOURCES = ["phantom.hip", "radon.hip", "filter.hip", "backproject.hip", "capi.cpp"]
+SOURCES = ["phantom.hip", "radon.hip", "filter.hip", "backproject.hip", "pointcloud.hip",
+           "filter_gemm.hip", "capi.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -51,7 +52,25 @@ def build(jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+HOST = os.path.join(PKG, "host")
+DRIVER = os.path.join(PKG, "hip_backend_driver")
+
+
+def build_driver() -> str:
+    """The C++ IReconstructionBackend adapter + its driver (host code only, g++)."""
+    srcs = [os.path.join(HOST, f) for f in ("HipBackend.cpp", "BackendFactory.cpp",
+                                             "hip_backend_driver.cpp")]
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", DRIVER + ".tmp", *srcs,
+           "-L", PKG, "-lct_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

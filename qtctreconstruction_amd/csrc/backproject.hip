@@ -1,159 +1,140 @@
-// backproject.hip — voxel-driven backprojection with LDS-staged projections.
+// backproject.hip — voxel-driven backprojection kernels (the tile routine and
+// its numerics are in bp_tile.h).
 //
-// Reference semantics (SerialBackend.cpp:197-252): for voxel (x,y) of an
-// N x N slice, xx = x - R, yy = y - R with R = float(N/2); voxels with
-// xx^2 + yy^2 > R^2 are 0; otherwise for every angle
-//   u = (xx*cos - yy*sin) + float(sq/2),  i0 = floor(u),  fr = u - i0,
-//   sum += f[i0]*(1-fr) + f[i0+1]*fr   when i0 >= 0 and i0+1 < sq,
-// and the sum is scaled by pi/(2 na). u, i0 and fr are computed here with the
-// same float operations (contraction off), so detector indexing is
-// bit-identical to Serial's; the interpolation uses one FMA
-// f[i0] + fr*(f[i0+1]-f[i0]) and the angle sum runs in Serial's order.
+//   bp_kernel<T,A,CHECK>          one volume, grid (tiles_x, tiles_y, z-blocks)
+//   bp_persistent_kernel<T,A>     many volumes of uneven size in one launch:
+//                                 work items (job, z-block, tile) are taken from
+//                                 an atomic counter until the list is exhausted
 //
-// Work split: a block owns a 32x32 voxel tile of one z-block (4 slices); each
-// of its 256 threads owns 4 voxels x 4 slices = 16 accumulators. Angles are
-// processed in chunks of kA: for every angle of the chunk the block stages
-// into LDS the detector window the tile can touch (kF bins starting at
-// floor(min corner u) - 1; u is monotone in x and y, so the corners bound it)
-// as (f, f[i+1]-f[i]) pairs of float4, i.e. both interpolation operands for
-// 4 slices in two 16-byte LDS reads. The trig tables are read with
-// wave-uniform indices, i.e. through the scalar cache.
-#include "ct_internal.h"
+// (T, A) = (voxel tile edge, angles per LDS stage) is a tuning choice; the
+// instantiated set is kBpConfigs and the autotuner picks among them.
+#include "bp_tile.h"
 
 namespace cthip {
 
-constexpr int kT = 32;   // tile edge (voxels)
-constexpr int kA = 32;   // angles per LDS stage
-constexpr int kF = 48;   // staged bins per angle: ceil(31*sqrt2) + 4
+template <int T, int A, bool CHECK>
+__global__ __launch_bounds__(256) void bp_kernel(BpArgs p) {
+    __shared__ BpSmem<T, A> sm;
+    bp_tile<T, A, CHECK>(p, blockIdx.x * T, blockIdx.y * T, blockIdx.z, sm);
+}
 
-template <bool CHECK>
-__global__ __launch_bounds__(256) void bp_kernel(const float4* __restrict__ filt, int nz, int na,
-                                                 int sq, int N, const float* __restrict__ cos_t,
-                                                 const float* __restrict__ sin_t, float scale,
-                                                 const float* __restrict__ mm,
-                                                 float* __restrict__ out) {
-    __shared__ float4 sf[kA][kF];
-    __shared__ float4 sd[kA][kF];
-    __shared__ int slo[kA];
-    const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
-    const int bx = blockIdx.x * kT, by = blockIdx.y * kT, zb = blockIdx.z;
-    const float R = (float)(N / 2);
-    const float dcb = (float)(sq / 2);
-    // this thread's voxels: (bx + tx + 16*i, by + ty + 16*j), i,j in {0,1}
-    const float xx0 = (float)(bx + tx) - R, xx1 = (float)(bx + tx + 16) - R;
-    const float yy0 = (float)(by + ty) - R, yy1 = (float)(by + ty + 16) - R;
-    const float cxa = (float)bx - R, cxb = (float)(bx + kT - 1) - R;
-    const float cya = (float)by - R, cyb = (float)(by + kT - 1) - R;
-    float4 acc00 = make_float4(0.f, 0.f, 0.f, 0.f), acc01 = acc00, acc10 = acc00, acc11 = acc00;
-    const float4* __restrict__ fz = filt + (size_t)zb * na * sq;
+template <int T, int A>
+__global__ __launch_bounds__(256) void bp_persistent_kernel(const BpJob* __restrict__ jobs,
+                                                            const int* __restrict__ hdr,
+                                                            int* __restrict__ counter) {
+    __shared__ BpSmem<T, A> sm;
+    __shared__ int item_s;
+    const int njobs = hdr[0], total = hdr[1];
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
+        __syncthreads();
+        const int item = item_s;
+        __syncthreads();
+        if (item >= total) break;
+        int j = 0;
+        while (j + 1 < njobs && jobs[j + 1].item_begin <= item) ++j;
+        const BpJob& jb = jobs[j];
+        const int tx = (jb.a.N + T - 1) / T;
+        const int per = tx * tx;
+        int local = item - jb.item_begin;
+        const int zb = local / per;
+        local -= zb * per;
+        const int by = local / tx, bx = local - by * tx;
+        if (jb.a.check)
+            bp_tile<T, A, true>(jb.a, bx * T, by * T, zb, sm);
+        else
+            bp_tile<T, A, false>(jb.a, bx * T, by * T, zb, sm);
+    }
+}
 
-    for (int a0 = 0; a0 < na; a0 += kA) {
-        const int nc = min(kA, na - a0);
-        if (tid < nc) {
-            const float c = cos_t[a0 + tid], s = sin_t[a0 + tid];
-            const float u00 = (cxa * c - cya * s) + dcb;
-            const float u01 = (cxb * c - cya * s) + dcb;
-            const float u10 = (cxa * c - cyb * s) + dcb;
-            const float u11 = (cxb * c - cyb * s) + dcb;
-            const float um = fminf(fminf(u00, u01), fminf(u10, u11));
-            slo[tid] = (int)floorf(um) - 1;
-        }
-        __syncthreads();
-        for (int e = tid; e < nc * kF; e += 256) {
-            const int aa = e / kF, j = e - aa * kF;
-            const int g = slo[aa] + j;
-            const float4* row = fz + (size_t)(a0 + aa) * sq;
-            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            const float4 f0 = (g >= 0 && g < sq) ? row[g] : z4;
-            const float4 f1 = (g + 1 >= 0 && g + 1 < sq) ? row[g + 1] : z4;
-            sf[aa][j] = f0;
-            sd[aa][j] = make_float4(f1.x - f0.x, f1.y - f0.y, f1.z - f0.z, f1.w - f0.w);
-        }
-        __syncthreads();
-        for (int aa = 0; aa < nc; ++aa) {
-            const float c = cos_t[a0 + aa], s = sin_t[a0 + aa];
-            const int lo = slo[aa];
-            const float xc0 = xx0 * c, xc1 = xx1 * c;
-            const float ys0 = yy0 * s, ys1 = yy1 * s;
-#define CT_BP_VOXEL(XC, YS, ACC)                                                   \
-    {                                                                              \
-        const float t = XC - YS;                                                   \
-        const float u = t + dcb;                                                   \
-        const float fl = floorf(u);                                                \
-        const int i0 = (int)fl;                                                    \
-        const float fr = u - fl;                                                   \
-        const int j = min(max(i0 - lo, 0), kF - 1);                                \
-        const float4 f = sf[aa][j];                                                \
-        const float4 d = sd[aa][j];                                                \
-        if (!CHECK || (i0 >= 0 && i0 + 1 < sq)) {                                  \
-            ACC.x += fmaf(fr, d.x, f.x);                                           \
-            ACC.y += fmaf(fr, d.y, f.y);                                           \
-            ACC.z += fmaf(fr, d.z, f.z);                                           \
-            ACC.w += fmaf(fr, d.w, f.w);                                           \
-        }                                                                          \
-    }
-            CT_BP_VOXEL(xc0, ys0, acc00)
-            CT_BP_VOXEL(xc1, ys0, acc10)
-            CT_BP_VOXEL(xc0, ys1, acc01)
-            CT_BP_VOXEL(xc1, ys1, acc11)
-#undef CT_BP_VOXEL
-        }
-        __syncthreads();
-    }
+__global__ void zero_ints_kernel(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
 
-    const size_t ss = (size_t)N * N;
-    float lo_z[kZB], eff_z[kZB];
-#pragma unroll
-    for (int k = 0; k < kZB; ++k) {
-        const int z = zb * kZB + k;
-        lo_z[k] = 0.0f;
-        eff_z[k] = 1.0f;
-        if (mm && z < nz) {
-            const float span = mm[2 * z + 1] - mm[2 * z];
-            lo_z[k] = mm[2 * z];
-            eff_z[k] = (span <= 0.0f) ? 1.0f : span;  // SerialBackend.cpp:298-304
-        }
-    }
-    const float4 accs[2][2] = {{acc00, acc01}, {acc10, acc11}};
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-#pragma unroll
-        for (int jj = 0; jj < 2; ++jj) {
-            const int x = bx + tx + 16 * i, y = by + ty + 16 * jj;
-            if (x >= N || y >= N) continue;
-            const float xx = (float)x - R, yy = (float)y - R;
-            const bool inside = !(xx * xx + yy * yy > R * R);
-            const float4 av = accs[i][jj];
-            const float v4[4] = {av.x, av.y, av.z, av.w};
-#pragma unroll
-            for (int k = 0; k < kZB; ++k) {
-                const int z = zb * kZB + k;
-                if (z >= nz) break;
-                float v = inside ? v4[k] * scale : 0.0f;
-                if (mm) v = v * eff_z[k] + lo_z[k];
-                out[(size_t)z * ss + (size_t)y * N + x] = v;
-            }
-        }
+const BpConfig kBpConfigs[kNumBpConfigs] = {{32, 32}, {32, 16}, {32, 64}, {16, 32}, {16, 64}};
+
+template <int T, int A>
+static hipError_t launch_one(const BpArgs& p, hipStream_t s) {
+    const int t = (p.N + T - 1) / T;
+    dim3 grid(t, t, (p.nz + kZB - 1) / kZB);
+    if (p.check)
+        hipLaunchKernelGGL((bp_kernel<T, A, true>), grid, dim3(256), 0, s, p);
+    else
+        hipLaunchKernelGGL((bp_kernel<T, A, false>), grid, dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_backproject_cfg(int cfg, const BpArgs& p, hipStream_t s) {
+    switch (cfg) {
+        case 0: return launch_one<32, 32>(p, s);
+        case 1: return launch_one<32, 16>(p, s);
+        case 2: return launch_one<32, 64>(p, s);
+        case 3: return launch_one<16, 32>(p, s);
+        case 4: return launch_one<16, 64>(p, s);
+        default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_backproject(const float4* d_filt, int nz, int na, int sq, int N,
                               const float* d_cos, const float* d_sin, float scale,
-                              const float* d_mm, float* d_out, hipStream_t s) {
-    const int nzb = (nz + kZB - 1) / kZB;
-    dim3 grid((N + kT - 1) / kT, (N + kT - 1) / kT, nzb);
-    // Every voxel inside the circle |(xx,yy)| <= R has |u - sq/2| <= R(1+1e-6),
-    // so when [sq/2 - R - 1, sq/2 + R + 1] lies in [0, sq-2] no bounds test is
-    // needed (always true for the reference geometry output_size == bins).
-    const double R = (double)(N / 2), dcb = (double)(sq / 2);
-    const bool safe = (dcb - R - 1.0 >= 0.0) && (dcb + R + 1.0 <= (double)sq - 2.0);
-    if (safe)
-        hipLaunchKernelGGL(bp_kernel<false>, grid, dim3(256), 0, s, d_filt, nz, na, sq, N, d_cos,
-                           d_sin, scale, d_mm, d_out);
-    else
-        hipLaunchKernelGGL(bp_kernel<true>, grid, dim3(256), 0, s, d_filt, nz, na, sq, N, d_cos,
-                           d_sin, scale, d_mm, d_out);
+                              const float* d_mm, float* d_out, hipStream_t s, int cfg) {
+    BpArgs p;
+    p.filt = d_filt;
+    p.out = d_out;
+    p.cos_t = d_cos;
+    p.sin_t = d_sin;
+    p.mm = d_mm;
+    p.nz = nz;
+    p.na = na;
+    p.sq = sq;
+    p.N = N;
+    p.scale = scale;
+    p.check = bp_needs_check(N, sq) ? 1 : 0;
+    return launch_backproject_cfg(cfg, p, s);
+}
+
+template <int T, int A>
+static const void* persistent_fn() {
+    return (const void*)bp_persistent_kernel<T, A>;
+}
+
+static const void* bp_persistent_ptr(int cfg) {
+    switch (cfg) {
+        case 0: return persistent_fn<32, 32>();
+        case 1: return persistent_fn<32, 16>();
+        case 2: return persistent_fn<32, 64>();
+        case 3: return persistent_fn<16, 32>();
+        case 4: return persistent_fn<16, 64>();
+        default: return nullptr;
+    }
+}
+
+int bp_persistent_grid(int cfg, int device) {
+    const void* fn = bp_persistent_ptr(cfg);
+    if (!fn) return 0;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    return (per_cu < 1 ? 1 : per_cu) * cus;
+}
+
+hipError_t launch_bp_persistent(int cfg, const BpJob* d_jobs, const int* d_hdr, int* d_counter,
+                                int grid, hipStream_t s) {
+    switch (cfg) {
+        case 0: hipLaunchKernelGGL((bp_persistent_kernel<32, 32>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 1: hipLaunchKernelGGL((bp_persistent_kernel<32, 16>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 2: hipLaunchKernelGGL((bp_persistent_kernel<32, 64>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 3: hipLaunchKernelGGL((bp_persistent_kernel<16, 32>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 4: hipLaunchKernelGGL((bp_persistent_kernel<16, 64>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int* d_p, int n, hipStream_t s) {
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d_p, n);
     return hipGetLastError();
 }
 

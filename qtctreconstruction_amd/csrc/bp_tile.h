@@ -1,0 +1,157 @@
+// bp_tile.h — one backprojection tile, shared by the per-volume kernel and
+// the persistent multi-volume kernel.
+//
+// Reference semantics (SerialBackend.cpp:197-252): for voxel (x,y) of an
+// N x N slice, xx = x - R, yy = y - R with R = float(N/2); voxels with
+// xx^2 + yy^2 > R^2 are 0; otherwise for every angle
+//   u = (xx*cos - yy*sin) + float(sq/2),  i0 = floor(u),  fr = u - i0,
+//   sum += f[i0]*(1-fr) + f[i0+1]*fr   when i0 >= 0 and i0+1 < sq,
+// and the sum is scaled by pi/(2 na). u, i0 and fr are computed with the same
+// float operations (contraction off), so detector indexing is bit-identical
+// to Serial's; the interpolation is one FMA f[i0] + fr*(f[i0+1]-f[i0]) and
+// the angle sum runs in Serial's order.
+//
+// A tile is T x T voxels (T = 16 or 32) of one z-block (4 slices); its 256
+// threads form a 16 x 16 grid and each owns (T/16)^2 voxels x 4 slices.
+// Angles go in stages of A: per stage the block stages into LDS, for every
+// angle, the detector window the tile can touch (F bins from
+// floor(min corner u) - 1; u is monotone in x and y, so the corners bound it)
+// as (f, f[i+1]-f[i]) float4 pairs: both interpolation operands for 4 slices
+// in two 16-byte LDS reads. The trig tables are read with wave-uniform
+// indices, i.e. through the scalar cache.
+#pragma once
+
+#include "ct_internal.h"
+
+namespace cthip {
+
+// detector bins one angle of a T-tile can touch: ceil((T-1)*sqrt2) + 4
+__host__ __device__ constexpr int bp_window(int T) { return T == 32 ? 48 : (T == 16 ? 26 : 0); }
+
+template <int T, int A>
+struct BpSmem {
+    float4 f[A][bp_window(T)];
+    float4 d[A][bp_window(T)];
+    int lo[A];
+};
+
+template <int T, int A, bool CHECK>
+__device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
+                                        BpSmem<T, A>& sm) {
+    constexpr int F = bp_window(T);
+    constexpr int V = T / 16;  // voxels per thread per axis
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const float R = (float)(p.N / 2);
+    const float dcb = (float)(p.sq / 2);
+    float xx[V], yy[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        xx[i] = (float)(bx + tx + 16 * i) - R;
+        yy[i] = (float)(by + ty + 16 * i) - R;
+    }
+    const float cxa = (float)bx - R, cxb = (float)(bx + T - 1) - R;
+    const float cya = (float)by - R, cyb = (float)(by + T - 1) - R;
+    float4 acc[V][V];
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int j = 0; j < V; ++j) acc[i][j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4* __restrict__ fz = p.filt + (size_t)zb * p.na * p.sq;
+
+    for (int a0 = 0; a0 < p.na; a0 += A) {
+        const int nc = min(A, p.na - a0);
+        if (tid < nc) {
+            const float c = p.cos_t[a0 + tid], s = p.sin_t[a0 + tid];
+            const float u00 = (cxa * c - cya * s) + dcb;
+            const float u01 = (cxb * c - cya * s) + dcb;
+            const float u10 = (cxa * c - cyb * s) + dcb;
+            const float u11 = (cxb * c - cyb * s) + dcb;
+            sm.lo[tid] = (int)floorf(fminf(fminf(u00, u01), fminf(u10, u11))) - 1;
+        }
+        __syncthreads();
+        for (int e = tid; e < nc * F; e += 256) {
+            const int aa = e / F, j = e - aa * F;
+            const int g = sm.lo[aa] + j;
+            const float4* row = fz + (size_t)(a0 + aa) * p.sq;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float4 f0 = (g >= 0 && g < p.sq) ? row[g] : z4;
+            const float4 f1 = (g + 1 >= 0 && g + 1 < p.sq) ? row[g + 1] : z4;
+            sm.f[aa][j] = f0;
+            sm.d[aa][j] = make_float4(f1.x - f0.x, f1.y - f0.y, f1.z - f0.z, f1.w - f0.w);
+        }
+        __syncthreads();
+        for (int aa = 0; aa < nc; ++aa) {
+            const float c = p.cos_t[a0 + aa], s = p.sin_t[a0 + aa];
+            const int lo = sm.lo[aa];
+            float xc[V], ys[V];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                xc[i] = xx[i] * c;
+                ys[i] = yy[i] * s;
+            }
+#pragma unroll
+            for (int j = 0; j < V; ++j)
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    const float t = xc[i] - ys[j];
+                    const float u = t + dcb;
+                    const float fl = floorf(u);
+                    const int i0 = (int)fl;
+                    const float fr = u - fl;
+                    const int jj = min(max(i0 - lo, 0), F - 1);
+                    const float4 f = sm.f[aa][jj];
+                    const float4 d = sm.d[aa][jj];
+                    if (!CHECK || (i0 >= 0 && i0 + 1 < p.sq)) {
+                        acc[i][j].x += fmaf(fr, d.x, f.x);
+                        acc[i][j].y += fmaf(fr, d.y, f.y);
+                        acc[i][j].z += fmaf(fr, d.z, f.z);
+                        acc[i][j].w += fmaf(fr, d.w, f.w);
+                    }
+                }
+        }
+        __syncthreads();
+    }
+
+    const size_t ss = (size_t)p.N * p.N;
+    float lo_z[kZB], eff_z[kZB];
+#pragma unroll
+    for (int k = 0; k < kZB; ++k) {
+        const int z = zb * kZB + k;
+        lo_z[k] = 0.0f;
+        eff_z[k] = 1.0f;
+        if (p.mm && z < p.nz) {
+            const float span = p.mm[2 * z + 1] - p.mm[2 * z];
+            lo_z[k] = p.mm[2 * z];
+            eff_z[k] = (span <= 0.0f) ? 1.0f : span;  // SerialBackend.cpp:298-304
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int j = 0; j < V; ++j) {
+            const int x = bx + tx + 16 * i, y = by + ty + 16 * j;
+            if (x >= p.N || y >= p.N) continue;
+            const float xf = (float)x - R, yf = (float)y - R;
+            const bool inside = !(xf * xf + yf * yf > R * R);
+            const float v4[4] = {acc[i][j].x, acc[i][j].y, acc[i][j].z, acc[i][j].w};
+#pragma unroll
+            for (int k = 0; k < kZB; ++k) {
+                const int z = zb * kZB + k;
+                if (z >= p.nz) break;
+                float v = inside ? v4[k] * p.scale : 0.0f;
+                if (p.mm) v = v * eff_z[k] + lo_z[k];
+                p.out[(size_t)z * ss + (size_t)y * p.N + x] = v;
+            }
+        }
+}
+
+// Every voxel inside the circle |(xx,yy)| <= R has |u - sq/2| <= R(1+1e-6),
+// so when [sq/2 - R - 1, sq/2 + R + 1] lies in [0, sq-2] no bounds test is
+// needed (always true for the reference geometry output_size == bins).
+inline bool bp_needs_check(int N, int sq) {
+    const double R = (double)(N / 2), dcb = (double)(sq / 2);
+    return !((dcb - R - 1.0 >= 0.0) && (dcb + R + 1.0 <= (double)sq - 2.0));
+}
+
+}  // namespace cthip

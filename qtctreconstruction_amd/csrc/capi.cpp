@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <new>
 #include <stdexcept>
@@ -42,14 +43,27 @@ struct ct_hip_ctx {
     std::string err;
     std::unordered_map<void*, size_t> sizes;  // every live device allocation
     // workspaces (grow-only, reused across calls)
-    Buf vol, row, col, sino, filt, out, keys, mm, ell;
+    Buf vol, row, col, sino, filt, out, keys, mm, ell, pc_cnt, pc_pts, pc_coord;
     // cached device state
     std::map<std::pair<int, int>, float*> filters;  // (nfft, type) -> H[nfft/2+1]
     std::map<int, float2*> twiddles;                // nfft -> tw[nfft/2]
-    std::vector<uint32_t> trig_key;                 // bit pattern of the angle list
-    Buf trig_cos, trig_sin;
+    struct Trig {
+        float* cos = nullptr;
+        float* sin = nullptr;
+    };
+    std::map<std::vector<uint32_t>, Trig> trig;  // keyed by the angle list's bit pattern
+    Trig cur;                                     // table of the current plain call
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double timings[4] = {0, 0, 0, 0};
+    int filter_backend = 0;  // CT_FILTER_BACKEND_*
+    struct Toep {
+        int Np = 0, Kp = 0;
+        unsigned short *hi = nullptr, *lo = nullptr;
+        unsigned char* f8 = nullptr;
+        float* tmax = nullptr;
+    };
+    std::map<std::tuple<int, int, int>, Toep> toeplitz;  // (nfft, type, bins)
+    Buf rmax;
 };
 
 namespace {
@@ -113,25 +127,29 @@ void release(ct_hip_ctx* c, Buf& b) {
 // then stays resident and is read through the scalar cache.
 float deg2rad(float d) { return d * 3.14159265358979323846f / 180.0f; }
 
-void set_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
+ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
     std::vector<uint32_t> key(na);
     std::memcpy(key.data(), angles_deg, sizeof(float) * na);
-    if (key == c->trig_key && c->trig_cos.p) return;
+    auto it = c->trig.find(key);
+    if (it != c->trig.end()) return it->second;
     std::vector<float> cs(na), sn(na);
     for (int a = 0; a < na; ++a) {
         const float th = deg2rad(angles_deg[a]);
         cs[a] = std::cos(th);
         sn[a] = std::sin(th);
     }
-    float* dc = ensure<float>(c, c->trig_cos, sizeof(float) * na);
-    float* ds = ensure<float>(c, c->trig_sin, sizeof(float) * na);
-    hip_check(hipMemcpyAsync(dc, cs.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
+    ct_hip_ctx::Trig t;
+    t.cos = static_cast<float*>(dmalloc(c, sizeof(float) * na));
+    t.sin = static_cast<float*>(dmalloc(c, sizeof(float) * na));
+    hip_check(hipMemcpyAsync(t.cos, cs.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
               "upload cos table");
-    hip_check(hipMemcpyAsync(ds, sn.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
+    hip_check(hipMemcpyAsync(t.sin, sn.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
               "upload sin table");
     hip_check(hipStreamSynchronize(c->stream), "trig upload");
-    c->trig_key = std::move(key);
+    return c->trig[std::move(key)] = t;
 }
+
+void set_trig(ct_hip_ctx* c, const float* angles_deg, int na) { c->cur = get_trig(c, angles_deg, na); }
 
 std::vector<float> serial_angles(int na) {
     std::vector<float> a(na);
@@ -197,27 +215,71 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
     hip_check(launch_normalise(d_vol, w, h, nz, d_mm, row, col, c->stream), "normalise");
-    hip_check(launch_radon(row, col, w, h, nz, na, bins, (const float*)c->trig_cos.p,
-                           (const float*)c->trig_sin.p, d_sino, c->stream),
+    hip_check(launch_radon(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin, d_sino, c->stream),
               "radon");
+}
+
+const ct_hip_ctx::Toep& get_toeplitz(ct_hip_ctx* c, const Geom& g, int type) {
+    type = norm_filter(type);
+    const auto key = std::make_tuple(g.nfft, type, g.bins);
+    auto it = c->toeplitz.find(key);
+    if (it != c->toeplitz.end()) return it->second;
+    ct_hip_ctx::Toep t;
+    t.Np = (g.sq + 255) / 256 * 256;
+    t.Kp = (g.bins + 31) / 32 * 32;
+    const size_t n = (size_t)t.Np * t.Kp;
+    t.hi = static_cast<unsigned short*>(dmalloc(c, 2 * n));
+    t.lo = static_cast<unsigned short*>(dmalloc(c, 2 * n));
+    t.f8 = static_cast<unsigned char*>(dmalloc(c, n + 8));
+    t.tmax = static_cast<float*>(dmalloc(c, sizeof(float)));
+    double* H = static_cast<double*>(dmalloc(c, sizeof(double) * (g.nfft / 2 + 1)));
+    double* h = static_cast<double*>(dmalloc(c, sizeof(double) * g.nfft));
+    float* T32 = static_cast<float*>(dmalloc(c, sizeof(float) * n));
+    hip_check(launch_toeplitz_design(g.nfft, type, g.sq, g.bins, g.pad, t.Np, t.Kp, H, h, t.hi,
+                                     t.lo, T32, t.f8, t.tmax, c->stream),
+              "toeplitz design");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    dfree(c, H);
+    dfree(c, h);
+    dfree(c, T32);
+    return c->toeplitz[key] = t;
+}
+
+// row filter of nz sinograms [nz][na][bins] -> filt [nz/4][na][sq][4]
+void stage_filter(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int filter,
+                  float4* filt) {
+    const Geom g = geometry(bins);
+    if (c->filter_backend == CT_FILTER_BACKEND_FFT_F32) {
+        check_fft(g);
+        const float* H = get_filter(c, g.nfft, filter);
+        const float2* tw = get_twiddles(c, g.nfft);
+        hip_check(launch_row_filter_fft(d_sino, nz, na, bins, g.sq, g.pad, g.nfft, H, tw, filt,
+                                        c->stream),
+                  "row filter");
+        return;
+    }
+    const ct_hip_ctx::Toep& t = get_toeplitz(c, g, filter);
+    float* rmax = nullptr;
+    if (c->filter_backend == CT_FILTER_BACKEND_GEMM_FP8) {
+        rmax = ensure<float>(c, c->rmax, sizeof(float) * (size_t)nz * na);
+        hip_check(launch_row_absmax(d_sino, (size_t)nz * na, bins, rmax, c->stream), "row absmax");
+    }
+    hip_check(launch_toeplitz_gemm(c->filter_backend, d_sino, nz, na, bins, g.sq, t.Np, t.Kp, t.hi,
+                                   t.lo, t.f8, rmax, t.tmax, filt, c->stream),
+              "toeplitz gemm");
 }
 
 // filter + backprojection of nz sinograms [nz][na][bins] onto N x N
 void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int N, int filter,
                const float* d_mm, float* d_out) {
     const Geom g = geometry(bins);
-    check_fft(g);
-    const float* H = get_filter(c, g.nfft, filter);
-    const float2* tw = get_twiddles(c, g.nfft);
+    if (c->filter_backend == CT_FILTER_BACKEND_FFT_F32) check_fft(g);
     float4* filt = ensure<float4>(c, c->filt, sizeof(float4) * nzb_of(nz) * (size_t)na * g.sq);
     hip_check(hipEventRecord(c->ev[1], c->stream), "event");
-    hip_check(launch_row_filter_fft(d_sino, nz, na, bins, g.sq, g.pad, g.nfft, H, tw, filt,
-                                    c->stream),
-              "row filter");
+    stage_filter(c, d_sino, nz, na, bins, filter, filt);
     hip_check(hipEventRecord(c->ev[2], c->stream), "event");
     const float scale = 3.14159265358979323846f / (2.0f * (float)na);  // SerialBackend.cpp:246
-    hip_check(launch_backproject(filt, nz, na, g.sq, N, (const float*)c->trig_cos.p,
-                                 (const float*)c->trig_sin.p, scale, d_mm, d_out, c->stream),
+    hip_check(launch_backproject(filt, nz, na, g.sq, N, c->cur.cos, c->cur.sin, scale, d_mm, d_out, c->stream),
               "backprojection");
     hip_check(hipEventRecord(c->ev[3], c->stream), "event");
 }
@@ -387,10 +449,23 @@ int ct_hip_clear_workspace(ct_hip_ctx* c) {
     return guarded(c, [&] {
         hip_check(hipStreamSynchronize(c->stream), "sync");
         for (Buf* b : {&c->vol, &c->row, &c->col, &c->sino, &c->filt, &c->out, &c->keys, &c->mm,
-                       &c->ell, &c->trig_cos, &c->trig_sin})
+                       &c->ell, &c->pc_cnt, &c->pc_pts, &c->pc_coord})
             release(c, *b);
-        c->trig_key.clear();
+        for (auto& kv : c->trig) {
+            dfree(c, kv.second.cos);
+            dfree(c, kv.second.sin);
+        }
+        c->trig.clear();
+        c->cur = ct_hip_ctx::Trig();
         for (auto& kv : c->filters) dfree(c, kv.second);
+        for (auto& kv : c->toeplitz) {
+            dfree(c, kv.second.hi);
+            dfree(c, kv.second.lo);
+            dfree(c, kv.second.f8);
+            dfree(c, kv.second.tmax);
+        }
+        c->toeplitz.clear();
+        release(c, c->rmax);
         for (auto& kv : c->twiddles) dfree(c, kv.second);
         c->filters.clear();
         c->twiddles.clear();
@@ -601,6 +676,79 @@ int ct_hip_reconstruct_from_sinograms(ct_hip_ctx* c, const float* sinos, int nz,
     });
 }
 
+static void point_cloud_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d,
+                            const float* xc, const float* yc, const float* zc, float thr,
+                            float* out, size_t cap, size_t* count) {
+    need(d_vol && count, "extractPointCloud: null pointer");
+    need(w > 0 && h > 0 && d > 0, "extractPointCloud: empty input");
+    const size_t n = (size_t)w * h * d;
+    need(n < (1ull << 32), "extractPointCloud: volume too large");
+    const unsigned nb = pc_blocks(n);
+    unsigned* cnt = ensure<unsigned>(c, c->pc_cnt, sizeof(unsigned) * 2 * (size_t)nb);
+    hip_check(launch_pc_count(d_vol, n, thr, cnt, c->stream), "point count");
+    std::vector<unsigned> hc(nb);
+    hip_check(hipMemcpyAsync(hc.data(), cnt, sizeof(unsigned) * nb, hipMemcpyDeviceToHost,
+                             c->stream),
+              "download counts");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+    size_t total = 0;
+    for (unsigned b = 0; b < nb; ++b) {
+        const unsigned k = hc[b];
+        hc[b] = (unsigned)total;
+        total += k;
+    }
+    *count = total;
+    if (!out || cap < total || total == 0) return;
+    unsigned* offs = cnt + nb;
+    hip_check(hipMemcpyAsync(offs, hc.data(), sizeof(unsigned) * nb, hipMemcpyHostToDevice,
+                             c->stream),
+              "upload offsets");
+    float* coords = nullptr;
+    const float* dx = nullptr;
+    const float* dy = nullptr;
+    const float* dz = nullptr;
+    if (xc || yc || zc) {
+        coords = ensure<float>(c, c->pc_coord, sizeof(float) * ((size_t)w + h + d));
+        if (xc) {
+            hip_check(hipMemcpyAsync(coords, xc, sizeof(float) * w, hipMemcpyHostToDevice, c->stream), "xc");
+            dx = coords;
+        }
+        if (yc) {
+            hip_check(hipMemcpyAsync(coords + w, yc, sizeof(float) * h, hipMemcpyHostToDevice, c->stream), "yc");
+            dy = coords + w;
+        }
+        if (zc) {
+            hip_check(hipMemcpyAsync(coords + w + h, zc, sizeof(float) * d, hipMemcpyHostToDevice, c->stream), "zc");
+            dz = coords + w + h;
+        }
+    }
+    float4* pts = ensure<float4>(c, c->pc_pts, sizeof(float4) * total);
+    hip_check(launch_pc_write(d_vol, n, w, h, thr, offs, dx, dy, dz, pts, c->stream), "point write");
+    hip_check(hipMemcpyAsync(out, pts, sizeof(float4) * total, hipMemcpyDeviceToHost, c->stream),
+              "download points");
+    hip_check(hipStreamSynchronize(c->stream), "sync");
+}
+
+int ct_hip_extract_point_cloud_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d,
+                                   const float* xc, const float* yc, const float* zc,
+                                   float threshold, float* out, size_t cap, size_t* count) {
+    return guarded(c, [&] { point_cloud_dev(c, d_vol, w, h, d, xc, yc, zc, threshold, out, cap, count); });
+}
+
+int ct_hip_extract_point_cloud(ct_hip_ctx* c, const float* vol, int w, int h, int d,
+                               const float* xc, const float* yc, const float* zc, float threshold,
+                               float* out, size_t cap, size_t* count) {
+    return guarded(c, [&] {
+        need(vol != nullptr && count != nullptr, "extractPointCloud: null pointer");
+        need(w > 0 && h > 0 && d > 0, "extractPointCloud: empty input");
+        const size_t n = (size_t)w * h * d;
+        float* d_vol = ensure<float>(c, c->vol, sizeof(float) * n);
+        hip_check(hipMemcpyAsync(d_vol, vol, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
+                  "upload volume");
+        point_cloud_dev(c, d_vol, w, h, d, xc, yc, zc, threshold, out, cap, count);
+    });
+}
+
 int ct_hip_design_filter(ct_hip_ctx* c, int n, int filter, float* out) {
     return guarded(c, [&] {
         need(out != nullptr, "designFilter: null output");
@@ -610,6 +758,40 @@ int ct_hip_design_filter(ct_hip_ctx* c, int n, int filter, float* out) {
                                  c->stream),
                   "download filter");
         hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int ct_hip_set_filter_backend(ct_hip_ctx* c, int backend) {
+    return guarded(c, [&] {
+        need(backend >= CT_FILTER_BACKEND_FFT_F32 && backend <= CT_FILTER_BACKEND_GEMM_FP8,
+             "setFilterBackend: unknown backend");
+        c->filter_backend = backend;
+    });
+}
+
+int ct_hip_filter_sinograms(ct_hip_ctx* c, const float* sinos, int nz, int na, int bins,
+                            int filter, float* out) {
+    return guarded(c, [&] {
+        need(sinos && out, "filterSinograms: null pointer");
+        need(nz > 0 && na > 0 && bins > 0, "filterSinograms: empty input");
+        const Geom g = geometry(bins);
+        const size_t nin = (size_t)nz * na * bins;
+        const size_t nf = nzb_of(nz) * (size_t)na * g.sq;
+        float* d_in = ensure<float>(c, c->sino, sizeof(float) * nin);
+        float4* filt = ensure<float4>(c, c->filt, sizeof(float4) * nf);
+        hip_check(hipMemcpyAsync(d_in, sinos, sizeof(float) * nin, hipMemcpyHostToDevice, c->stream),
+                  "upload sinograms");
+        stage_filter(c, d_in, nz, na, bins, filter, filt);
+        std::vector<float> tmp(4 * nf);
+        hip_check(hipMemcpyAsync(tmp.data(), filt, sizeof(float4) * nf, hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "download filtered");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        for (int z = 0; z < nz; ++z)
+            for (int a = 0; a < na; ++a)
+                for (int i = 0; i < g.sq; ++i)
+                    out[((size_t)z * na + a) * g.sq + i] =
+                        tmp[(((size_t)(z / kZB) * na + a) * g.sq + i) * kZB + (z % kZB)];
     });
 }
 

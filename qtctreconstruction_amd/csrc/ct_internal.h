@@ -50,6 +50,24 @@ hipError_t launch_radon(const float4* d_row, const float4* d_col, int w, int h, 
                         hipStream_t s);
 
 // ---- row filter ----------------------------------------------------------
+struct FiltArgs {
+    const float* sino;  // [nz][na][bins]
+    int nz, na, bins, sq, pad, nfft, logn;
+    const float* H;     // nfft/2+1 taps
+    const float2* tw;   // nfft/2 twiddles
+    float4* filt;       // [nzb][na][sq] float4
+};
+struct FiltJob {
+    FiltArgs a;
+    int item_begin;  // first work item of this job; items are (z-block, angle)
+    int pad_;
+};
+FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, int pad, int nfft,
+                        const float* d_H, const float2* d_tw, float4* d_filt);
+size_t fft_lds_bytes(int nfft);
+// hdr = {njobs, total items}; counter must be 0 at launch
+hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
+                                        int max_nfft, int grid, hipStream_t s);
 hipError_t launch_filter_design(int n, int type, float* d_H, hipStream_t s);
 hipError_t launch_twiddles(int n, float2* d_tw, hipStream_t s);
 // sino [nz][na][bins] -> filt [nzb][na][sq] float4 (4 slices interleaved)
@@ -57,10 +75,58 @@ hipError_t launch_row_filter_fft(const float* d_sino, int nz, int na, int bins, 
                                  int nfft, const float* d_H, const float2* d_tw, float4* d_filt,
                                  hipStream_t s);
 
+// Toeplitz-GEMM row filter (MFMA). T is Np x Kp (Np = sq rounded up to 256,
+// Kp = bins rounded up to 32); variant 1 BF16, 2 split BF16, 3 FP8 E4M3.
+hipError_t launch_toeplitz_design(int nfft, int type, int sq, int bins, int pad, int Np, int Kp,
+                                  double* d_H, double* d_h, unsigned short* d_Thi,
+                                  unsigned short* d_Tlo, float* d_T32, unsigned char* d_T8,
+                                  float* d_tmax, hipStream_t s);
+hipError_t launch_row_absmax(const float* d_sino, size_t rows, int cols, float* d_rmax,
+                             hipStream_t s);
+hipError_t launch_toeplitz_gemm(int variant, const float* d_sino, int nz, int na, int bins,
+                                int sq, int Np, int Kp, const unsigned short* d_Thi,
+                                const unsigned short* d_Tlo, const unsigned char* d_T8,
+                                const float* d_rmax, const float* d_tmax, float4* d_filt,
+                                hipStream_t s);
+
 // ---- backprojection --------------------------------------------------------
+struct BpArgs {
+    const float4* filt;  // [nzb][na][sq] float4 (4 slices)
+    float* out;          // [nz][N][N]
+    const float* cos_t;
+    const float* sin_t;
+    const float* mm;  // optional {min, max} per slice -> denormalise
+    int nz, na, sq, N;
+    float scale;
+    int check;  // 1: test i0 >= 0 && i0+1 < sq per update
+};
+struct BpJob {
+    BpArgs a;
+    int item_begin;  // first work item; items are (z-block, tile) in z-major order
+    int pad_;
+};
+struct BpConfig {
+    int tile;    // voxel tile edge (16 or 32)
+    int angles;  // angles per LDS stage
+};
+constexpr int kNumBpConfigs = 5;
+extern const BpConfig kBpConfigs[kNumBpConfigs];
 // out[z][y][x] = inside ? acc*scale : 0, then (if d_mm) * eff_span + min.
 hipError_t launch_backproject(const float4* d_filt, int nz, int na, int sq, int N,
                               const float* d_cos, const float* d_sin, float scale,
-                              const float* d_mm, float* d_out, hipStream_t s);
+                              const float* d_mm, float* d_out, hipStream_t s, int cfg = 0);
+hipError_t launch_backproject_cfg(int cfg, const BpArgs& p, hipStream_t s);
+int bp_persistent_grid(int cfg, int device);
+hipError_t launch_bp_persistent(int cfg, const BpJob* d_jobs, const int* d_hdr, int* d_counter,
+                                int grid, hipStream_t s);
+hipError_t launch_zero_ints(int* d_p, int n, hipStream_t s);
+
+// ---- point cloud ------------------------------------------------------------
+unsigned pc_blocks(size_t n);
+hipError_t launch_pc_count(const float* d_v, size_t n, float thr, unsigned* d_counts,
+                           hipStream_t s);
+hipError_t launch_pc_write(const float* d_v, size_t n, int w, int h, float thr,
+                           const unsigned* d_offs, const float* d_xc, const float* d_yc,
+                           const float* d_zc, float4* d_out, hipStream_t s);
 
 }  // namespace cthip

@@ -73,29 +73,28 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
-__global__ __launch_bounds__(256) void row_filter_fft_kernel(
-    const float* __restrict__ sino, int nz, int na, int bins, int sq, int pad, int n, int logn,
-    const float* __restrict__ Hh, const float2* __restrict__ tw, float4* __restrict__ filt) {
-    extern __shared__ float2 sm[];
+// One (angle, z-block) work item: 4 rows -> 2 complex FFTs of n points in LDS.
+__device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb,
+                                                float2* __restrict__ sm) {
+    const int n = p.nfft, logn = p.logn;
     float2* X0 = sm;
     float2* X1 = sm + n;
     float2* T = sm + 2 * n;
-    const int a = blockIdx.x;
-    const int zb = blockIdx.y;
     const int tid = threadIdx.x;
     const int nh = n / 2;
-    for (int k = tid; k < nh; k += 256) T[k] = tw[k];
+    for (int k = tid; k < nh; k += 256) T[k] = p.tw[k];
     const float* rows[kZB];
 #pragma unroll
     for (int r = 0; r < kZB; ++r) {
         const int z = zb * kZB + r;
-        rows[r] = (z < nz) ? sino + ((size_t)z * na + a) * bins : nullptr;
+        rows[r] = (z < p.nz) ? p.sino + ((size_t)z * p.na + a) * p.bins : nullptr;
     }
     for (int j = tid; j < n; j += 256) {
-        const int jj = j - pad;
+        const int jj = j - p.pad;
         float v[kZB];
 #pragma unroll
-        for (int r = 0; r < kZB; ++r) v[r] = (rows[r] && jj >= 0 && jj < bins) ? rows[r][jj] : 0.0f;
+        for (int r = 0; r < kZB; ++r)
+            v[r] = (rows[r] && jj >= 0 && jj < p.bins) ? rows[r][jj] : 0.0f;
         const int br = (int)(__brev((unsigned)j) >> (32 - logn));
         X0[br] = make_float2(v[0], v[1]);
         X1[br] = make_float2(v[2], v[3]);
@@ -120,7 +119,7 @@ __global__ __launch_bounds__(256) void row_filter_fft_kernel(
     }
     // multiply by the real, even filter (H[n-k] = H[k])
     for (int k = tid; k < n; k += 256) {
-        const float hv = Hh[k <= nh ? k : n - k];
+        const float hv = p.H[k <= nh ? k : n - k];
         X0[k] = make_float2(X0[k].x * hv, X0[k].y * hv);
         X1[k] = make_float2(X1[k].x * hv, X1[k].y * hv);
     }
@@ -143,12 +142,61 @@ __global__ __launch_bounds__(256) void row_filter_fft_kernel(
         __syncthreads();
     }
     const float inv_n = 1.0f / (float)n;
-    float4* dst = filt + ((size_t)zb * na + a) * sq;
-    for (int i = tid; i < sq; i += 256) {
+    float4* dst = p.filt + ((size_t)zb * p.na + a) * p.sq;
+    for (int i = tid; i < p.sq; i += 256) {
         const int br = (int)(__brev((unsigned)i) >> (32 - logn));
-        const float2 p = X0[br], q = X1[br];
-        dst[i] = make_float4(p.x * inv_n, p.y * inv_n, q.x * inv_n, q.y * inv_n);
+        const float2 u = X0[br], v = X1[br];
+        dst[i] = make_float4(u.x * inv_n, u.y * inv_n, v.x * inv_n, v.y * inv_n);
     }
+    __syncthreads();  // LDS is reused by the next item
+}
+
+__global__ __launch_bounds__(256) void row_filter_fft_kernel(FiltArgs p) {
+    extern __shared__ float2 sm[];
+    fft_filter_item(p, blockIdx.x, blockIdx.y, sm);
+}
+
+// Persistent form: items (job, angle, z-block) of many volumes, taken from an
+// atomic counter until the list is exhausted (every block then returns).
+__global__ __launch_bounds__(256) void row_filter_persistent_kernel(const FiltJob* __restrict__ jobs,
+                                                                    const int* __restrict__ hdr,
+                                                                    int* __restrict__ counter) {
+    extern __shared__ float2 sm[];
+    __shared__ int item_s;
+    const int njobs = hdr[0], total = hdr[1];
+    for (;;) {
+        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
+        __syncthreads();
+        const int item = item_s;
+        __syncthreads();
+        if (item >= total) break;
+        int j = 0;
+        while (j + 1 < njobs && jobs[j + 1].item_begin <= item) ++j;
+        const FiltJob& jb = jobs[j];
+        const int local = item - jb.item_begin;
+        const int zb = local / jb.a.na, a = local - zb * jb.a.na;
+        fft_filter_item(jb.a, a, zb, sm);
+    }
+}
+
+size_t fft_lds_bytes(int nfft) { return sizeof(float2) * (size_t)(2 * nfft + nfft / 2); }
+
+static hipError_t set_lds_attr(const void* fn) {
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)fft_lds_bytes(kMaxFFT));
+}
+
+hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
+                                        int max_nfft, int grid, hipStream_t s) {
+    static bool attr = false;
+    if (!attr) {
+        const hipError_t e = set_lds_attr((const void*)row_filter_persistent_kernel);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    hipLaunchKernelGGL(row_filter_persistent_kernel, dim3(grid), dim3(256), fft_lds_bytes(max_nfft),
+                       s, d_jobs, d_hdr, d_counter);
+    return hipGetLastError();
 }
 
 hipError_t launch_filter_design(int n, int type, float* d_H, hipStream_t s) {
@@ -164,25 +212,37 @@ hipError_t launch_twiddles(int n, float2* d_tw, hipStream_t s) {
     return hipGetLastError();
 }
 
+FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, int pad, int nfft,
+                        const float* d_H, const float2* d_tw, float4* d_filt) {
+    FiltArgs p;
+    p.sino = d_sino;
+    p.nz = nz;
+    p.na = na;
+    p.bins = bins;
+    p.sq = sq;
+    p.pad = pad;
+    p.nfft = nfft;
+    p.logn = 0;
+    while ((1 << p.logn) < nfft) ++p.logn;
+    p.H = d_H;
+    p.tw = d_tw;
+    p.filt = d_filt;
+    return p;
+}
+
 hipError_t launch_row_filter_fft(const float* d_sino, int nz, int na, int bins, int sq, int pad,
                                  int nfft, const float* d_H, const float2* d_tw, float4* d_filt,
                                  hipStream_t s) {
     if (nfft < 4 || nfft > kMaxFFT || (nfft & (nfft - 1))) return hipErrorInvalidValue;
-    int logn = 0;
-    while ((1 << logn) < nfft) ++logn;
-    const size_t lds = sizeof(float2) * (size_t)(2 * nfft + nfft / 2);
     static bool attr_set = false;  // > 64 KB of dynamic LDS must be opted into
     if (!attr_set) {
-        const hipError_t e = hipFuncSetAttribute((const void*)row_filter_fft_kernel,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                 (int)(sizeof(float2) * (2 * kMaxFFT + kMaxFFT / 2)));
+        const hipError_t e = set_lds_attr((const void*)row_filter_fft_kernel);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
     const int nzb = (nz + kZB - 1) / kZB;
-    dim3 grid(na, nzb);
-    hipLaunchKernelGGL(row_filter_fft_kernel, grid, dim3(256), lds, s, d_sino, nz, na, bins, sq,
-                       pad, nfft, logn, d_H, d_tw, d_filt);
+    const FiltArgs p = make_filt_args(d_sino, nz, na, bins, sq, pad, nfft, d_H, d_tw, d_filt);
+    hipLaunchKernelGGL(row_filter_fft_kernel, dim3(na, nzb), dim3(256), fft_lds_bytes(nfft), s, p);
     return hipGetLastError();
 }
 

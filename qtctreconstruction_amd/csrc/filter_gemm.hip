@@ -1,0 +1,307 @@
+// filter_gemm.hip — the row filter as a Toeplitz GEMM on gfx950 MFMA.
+//
+// The padded FFT filter of SerialBackend.cpp:150-195 is exactly a linear map
+// of each row: q[i] = sum_j h[(i - j - pad) mod nfft] p[j] for i < sq and
+// j < bins, where h is the real-space kernel of the filter (DC and Nyquist
+// removed, the effective Serial filter). Stacking every (slice, angle) row
+// gives Q[M x sq] = P[M x bins] * T^T, a GEMM with T[n][k] = h[(n-k-pad) mod nfft].
+//
+// Variants (all accumulate in FP32):
+//   1  BF16:        Q = bf16(P) * bf16(T)
+//   2  split BF16:  Q = Ph*Th + Ph*Tl + Pl*Th, with X = Xh + Xl, Xh = bf16(X),
+//                   Xl = bf16(X - Xh)  (three products; error ~ 2^-16)
+//   3  FP8 E4M3:    per-row scale s_r = 448 / max|P_r|, one scale for T;
+//                   Q = (fp8(P*s_r) * fp8(T*s_T)) / (s_r s_T)
+//
+// Work split: a block owns 16 angles x 4 slices of one z-block (4 MFMA row
+// groups, one per slice) and 256 output bins; each of its 4 waves owns 64 bins
+// (4 MFMA 16x16 tiles). MFMA shapes: v_mfma_f32_16x16x32_bf16 and
+// v_mfma_f32_16x16x32_fp8_fp8. Per lane l (c = l%16, g = l/16): A holds row c,
+// k = 8g..8g+7; B holds column c, k = 8g..8g+7; D holds rows 4g..4g+3 of
+// column c. Because the 4 slice tiles of a (row, column) land in the same
+// lane, the epilogue writes float4 = 4 slices per bin, the layout the
+// backprojection stages from ([z/4][angle][sq][4]).
+//
+// The Toeplitz operand T is designed on the device in FP64 (H by direct DFT,
+// h by inverse DFT), then rounded to the variant's format, and stays resident.
+#include "ct_internal.h"
+
+namespace cthip {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f4v;
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    // round to nearest even (inputs are finite)
+    const unsigned u = __float_as_uint(f);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+
+// ---- design: H in FP64 (same formula as filter_design_kernel), then h ----
+__global__ void filter_design64_kernel(int n, int type, double* __restrict__ H) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nh = n / 2;
+    if (k > nh) return;
+    const double pi = 3.14159265358979323846;
+    double acc = 0.25;
+    for (int j = 1; j < n; j += 2) {
+        const int m = (j < nh) ? j : n - j;
+        const double t = 1.0 / (pi * (double)m);
+        const double fj = (double)(float)(-1.0 * t * t);
+        const long long r = ((long long)j * k) % n;
+        acc += fj * cos(2.0 * pi * (double)r / (double)n);
+    }
+    double h = 2.0 * acc;
+    const double freq = (double)k / (double)n;
+    switch (type) {
+        case 0: break;
+        case 2: h *= 0.54 + 0.46 * cos(pi * freq / 0.5); break;
+        case 3: h *= cos(pi * freq); break;
+        case 4: h *= 0.5 + 0.5 * cos(pi * freq / 0.5); break;
+        case 5: {
+            const double r = fabs(freq) / 0.5;
+            h *= 1.0 - (r < 1.0 ? r : 1.0);
+        } break;
+        case 1:
+        default: {
+            const double om = pi * freq;
+            if (k > 0) h *= sin(om) / om;
+        } break;
+    }
+    if (k == 0 || k == nh) h = 0.0;
+    H[k] = h;
+}
+
+// h[m] = (2/n) sum_{k=1}^{n/2-1} H[k] cos(2 pi k m / n)   (H[0] = H[n/2] = 0)
+__global__ void kernel_from_filter64(int n, const double* __restrict__ H, double* __restrict__ h) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const double pi = 3.14159265358979323846;
+    double acc = 0.0;
+    for (int k = 1; k < n / 2; ++k) {
+        const long long r = ((long long)k * m) % n;
+        acc += H[k] * cos(2.0 * pi * (double)r / (double)n);
+    }
+    h[m] = 2.0 * acc / (double)n;
+}
+
+// T[n][k] (Np x Kp, k contiguous): hi/lo bf16 and fp32 copies
+__global__ void toeplitz_build_kernel(const double* __restrict__ h, int nfft, int sq, int bins,
+                                      int pad, int Np, int Kp, unsigned short* __restrict__ Thi,
+                                      unsigned short* __restrict__ Tlo, float* __restrict__ T32) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)Np * Kp) return;
+    const int nn = (int)(idx / Kp), kk = (int)(idx - (size_t)nn * Kp);
+    float v = 0.0f;
+    if (nn < sq && kk < bins) {
+        int d = (nn - kk - pad) % nfft;
+        if (d < 0) d += nfft;
+        v = (float)h[d];
+    }
+    const unsigned short hi = f2bf(v);
+    Thi[idx] = hi;
+    Tlo[idx] = f2bf(v - bf2f(hi));
+    T32[idx] = v;
+}
+
+__global__ void absmax_rows_kernel(const float* __restrict__ x, size_t rows, int cols,
+                                   float* __restrict__ out) {
+    const size_t r = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    float m = 0.0f;
+    if (r < rows)
+        for (int c = lane; c < cols; c += 64) m = fmaxf(m, fabsf(x[r * cols + c]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if (r < rows && lane == 0) out[r] = m;
+}
+
+// single-block max over n values (T scale)
+__global__ void absmax_all_kernel(const float* __restrict__ x, size_t n, float* __restrict__ out) {
+    float m = 0.0f;
+    for (size_t i = threadIdx.x; i < n; i += blockDim.x) m = fmaxf(m, fabsf(x[i]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    __shared__ float s[16];
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fmaxf(m, s[i]);
+        out[0] = m;
+    }
+}
+
+// T quantised to FP8 E4M3 with scale 448/max|T|
+__global__ void toeplitz_fp8_kernel(const float* __restrict__ T32, size_t n,
+                                    const float* __restrict__ tmax, unsigned char* __restrict__ T8) {
+    const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 2;
+    if (i >= n) return;
+    const float s = (tmax[0] > 0.0f) ? 448.0f / tmax[0] : 1.0f;
+    const float a = T32[i] * s, b = (i + 1 < n) ? T32[i + 1] * s : 0.0f;
+    const int p = __builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false);
+    T8[i] = (unsigned char)(p & 0xff);
+    if (i + 1 < n) T8[i + 1] = (unsigned char)((p >> 8) & 0xff);
+}
+
+template <int VARIANT>
+__global__ __launch_bounds__(256) void toeplitz_gemm_kernel(
+    const float* __restrict__ sino, int nz, int na, int bins, int sq, int Kp,
+    const unsigned short* __restrict__ Thi, const unsigned short* __restrict__ Tlo,
+    const unsigned char* __restrict__ T8, const float* __restrict__ rmax,
+    const float* __restrict__ tmax, float4* __restrict__ filt) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int c = lane & 15, g = lane >> 4;
+    const int n0 = blockIdx.x * 256 + wave * 64;  // first output bin of this wave
+    const int a0 = blockIdx.y * 16;
+    const int zb = blockIdx.z;
+    const int a = a0 + c;  // A-operand row of this lane
+    f4v acc[kZB][4];
+#pragma unroll
+    for (int s = 0; s < kZB; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[s][t] = (f4v){0.f, 0.f, 0.f, 0.f};
+
+    const float* rowp[kZB];
+    float rs[kZB];
+#pragma unroll
+    for (int s = 0; s < kZB; ++s) {
+        const int z = zb * kZB + s;
+        const bool live = (z < nz) && (a < na);
+        rowp[s] = live ? sino + ((size_t)z * na + a) * bins : nullptr;
+        rs[s] = 1.0f;
+        if (VARIANT == 3 && live) {
+            const float m = rmax[(size_t)z * na + a];
+            rs[s] = (m > 0.0f) ? 448.0f / m : 1.0f;
+        }
+    }
+
+    for (int k0 = 0; k0 < Kp; k0 += 32) {
+        const int kb = k0 + 8 * g;
+        // B fragments for the wave's 4 column tiles
+        bf16x8 bh[4], bl[4];
+        long b8[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const size_t off = (size_t)(n0 + 16 * t + c) * Kp + kb;
+            if (VARIANT == 3) {
+                b8[t] = *reinterpret_cast<const long*>(T8 + off);
+            } else {
+                bh[t] = *reinterpret_cast<const bf16x8*>(Thi + off);
+                if (VARIANT == 2) bl[t] = *reinterpret_cast<const bf16x8*>(Tlo + off);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kZB; ++s) {
+            float x[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = kb + j;
+                x[j] = (rowp[s] && k < bins) ? rowp[s][k] : 0.0f;
+            }
+            if (VARIANT == 3) {
+                int lo = 0, hi = 0;
+                lo = __builtin_amdgcn_cvt_pk_fp8_f32(x[0] * rs[s], x[1] * rs[s], lo, false);
+                lo = __builtin_amdgcn_cvt_pk_fp8_f32(x[2] * rs[s], x[3] * rs[s], lo, true);
+                hi = __builtin_amdgcn_cvt_pk_fp8_f32(x[4] * rs[s], x[5] * rs[s], hi, false);
+                hi = __builtin_amdgcn_cvt_pk_fp8_f32(x[6] * rs[s], x[7] * rs[s], hi, true);
+                const long av = (long)(unsigned)lo | ((long)(unsigned)hi << 32);
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    acc[s][t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, b8[t], acc[s][t], 0, 0, 0);
+            } else {
+                bf16x8 ah, al;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const unsigned short hb = f2bf(x[j]);
+                    ah[j] = __builtin_bit_cast(__bf16, hb);
+                    if (VARIANT == 2) al[j] = __builtin_bit_cast(__bf16, f2bf(x[j] - bf2f(hb)));
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    acc[s][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[s][t], 0, 0, 0);
+                    if (VARIANT == 2) {
+                        acc[s][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[t], acc[s][t], 0, 0, 0);
+                        acc[s][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[t], acc[s][t], 0, 0, 0);
+                    }
+                }
+            }
+        }
+    }
+
+    // epilogue: lane holds rows 4g..4g+3 (angles) of column c of every tile
+    float tsc = 1.0f;
+    if (VARIANT == 3) tsc = (tmax[0] > 0.0f) ? tmax[0] / 448.0f : 1.0f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int nn = n0 + 16 * t + c;
+        if (nn >= sq) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int aa = a0 + 4 * g + r;
+            if (aa >= na) continue;
+            float v[kZB];
+#pragma unroll
+            for (int s = 0; s < kZB; ++s) {
+                float q = acc[s][t][r];
+                if (VARIANT == 3) {
+                    const int z = zb * kZB + s;
+                    const float m = (z < nz) ? rmax[(size_t)z * na + aa] : 0.0f;
+                    q = q * ((m > 0.0f ? m / 448.0f : 1.0f) * tsc);
+                }
+                v[s] = q;
+            }
+            filt[((size_t)zb * na + aa) * sq + nn] = make_float4(v[0], v[1], v[2], v[3]);
+        }
+    }
+}
+
+hipError_t launch_toeplitz_design(int nfft, int type, int sq, int bins, int pad, int Np, int Kp,
+                                  double* d_H, double* d_h, unsigned short* d_Thi,
+                                  unsigned short* d_Tlo, float* d_T32, unsigned char* d_T8,
+                                  float* d_tmax, hipStream_t s) {
+    hipLaunchKernelGGL(filter_design64_kernel, dim3((nfft / 2 + 1 + 127) / 128), dim3(128), 0, s,
+                       nfft, type, d_H);
+    hipLaunchKernelGGL(kernel_from_filter64, dim3((nfft + 127) / 128), dim3(128), 0, s, nfft, d_H,
+                       d_h);
+    const size_t n = (size_t)Np * Kp;
+    hipLaunchKernelGGL(toeplitz_build_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       d_h, nfft, sq, bins, pad, Np, Kp, d_Thi, d_Tlo, d_T32);
+    hipLaunchKernelGGL(absmax_all_kernel, dim3(1), dim3(1024), 0, s, d_T32, n, d_tmax);
+    hipLaunchKernelGGL(toeplitz_fp8_kernel, dim3((unsigned)((n / 2 + 256) / 256)), dim3(256), 0, s,
+                       d_T32, n, d_tmax, d_T8);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_absmax(const float* d_sino, size_t rows, int cols, float* d_rmax,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(absmax_rows_kernel, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, s,
+                       d_sino, rows, cols, d_rmax);
+    return hipGetLastError();
+}
+
+hipError_t launch_toeplitz_gemm(int variant, const float* d_sino, int nz, int na, int bins,
+                                int sq, int Np, int Kp, const unsigned short* d_Thi,
+                                const unsigned short* d_Tlo, const unsigned char* d_T8,
+                                const float* d_rmax, const float* d_tmax, float4* d_filt,
+                                hipStream_t s) {
+    const int nzb = (nz + kZB - 1) / kZB;
+    dim3 grid(Np / 256, (na + 15) / 16, nzb);
+    switch (variant) {
+        case 1:
+            hipLaunchKernelGGL(toeplitz_gemm_kernel<1>, grid, dim3(256), 0, s, d_sino, nz, na, bins,
+                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt);
+            break;
+        case 2:
+            hipLaunchKernelGGL(toeplitz_gemm_kernel<2>, grid, dim3(256), 0, s, d_sino, nz, na, bins,
+                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt);
+            break;
+        case 3:
+            hipLaunchKernelGGL(toeplitz_gemm_kernel<3>, grid, dim3(256), 0, s, d_sino, nz, na, bins,
+                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace cthip

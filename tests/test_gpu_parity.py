@@ -186,3 +186,60 @@ def test_unsupported_width_is_an_error(backend):
         backend.reconstructSlice(ct.Sinogram(sino, np.zeros(4, np.float32)), 16,
                                  ct.ReconstructionParams())
     assert ei.value.code == ct.CT_ERR_UNSUPPORTED
+
+
+# ----------------------------------------------------------------- point cloud
+def _serial_point_cloud(vol, thr, xc=None, yc=None, zc=None):
+    """SerialBackend::extractPointCloud (SerialBackend.cpp:325-346) in numpy."""
+    d, h, w = vol.shape
+    z, y, x = np.nonzero(vol > thr)  # C order = z, y, x raster order
+    X = (np.asarray(xc, np.float32)[x] if xc is not None else x.astype(np.float32))
+    Y = (np.asarray(yc, np.float32)[y] if yc is not None else y.astype(np.float32))
+    Z = (np.asarray(zc, np.float32)[z] if zc is not None else z.astype(np.float32))
+    return np.stack([X, Y, Z, vol[z, y, x]], 1).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,thr", [((33, 40, 29), 0.3), ((64, 64, 64), 0.5), ((5, 7, 3), 10.0)])
+def test_point_cloud_order_and_values(backend, O, shape, thr):
+    d, h, w = shape
+    vol = O.phantom(31, 12, w, h, d)
+    got = backend.extractPointCloud(vol, thr)
+    ref = _serial_point_cloud(vol, thr)
+    assert got.shape == ref.shape and _bits_equal(got, ref)
+    xc = np.linspace(-1, 1, w).astype(np.float32)
+    zc = np.arange(d, dtype=np.float32) * 2.5
+    got = backend.extractPointCloud(vol, thr, x_coords=xc, z_coords=zc)
+    assert _bits_equal(got, _serial_point_cloud(vol, thr, xc=xc, zc=zc))
+
+
+# ----------------------------------------------------------------- C++ adapter
+def test_cpp_hip_backend_through_the_interface(O, tmp_path):
+    """HipBackend (C++) obtained from BackendFactory::createBestAvailable, called
+    through IReconstructionBackend as the controller calls it."""
+    import os
+    import subprocess
+    drv = os.path.join(os.path.dirname(ct.__file__), "hip_backend_driver")
+    w = h = 64
+    d, na, filt, thr = 6, 90, int(ct.FilterType.Hann), 0.4
+    vol = O.phantom(404, 10, w, h, d)
+    vol.tofile(tmp_path / "vol.f32")
+    r = subprocess.run([drv, str(tmp_path / "vol.f32"), str(w), str(h), str(d), str(na), str(filt),
+                        str(thr), str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    meta = (tmp_path / "meta.txt").read_text().split("\n")
+    assert meta[0] == "HIP (MI355X)" and int(meta[1]) == d and meta[3] == "1"
+    sino0 = np.fromfile(tmp_path / "sino0.f32", np.float32).reshape(na, w)
+    ref_s, _ = O.compute_sinogram(vol[0], na, w)
+    assert _bits_equal(sino0, ref_s)
+    sl = np.fromfile(tmp_path / "slice0.f32", np.float32).reshape(w, w)
+    assert O.rel_l2(sl, O.reconstruct_slice(ref_s, O.serial_angles(na), w, filt)) <= TOL
+    v = np.fromfile(tmp_path / "volume.f32", np.float32).reshape(d, h, w)
+    ref_v, _ = O.reconstruct_volume(vol, na, filt)
+    assert O.rel_l2(v, ref_v) <= TOL
+    fs = np.fromfile(tmp_path / "fromsino.f32", np.float32).reshape(d, w, w)
+    for z in range(d):
+        s, _ = O.compute_sinogram(vol[z], na, w)
+        assert O.rel_l2(fs[z], O.reconstruct_slice(s, O.serial_angles(na), w, filt)) <= TOL
+    pts = np.fromfile(tmp_path / "points.f32", np.float32).reshape(-1, 4)
+    xc = 0.5 * np.arange(w, dtype=np.float32)
+    assert _bits_equal(pts, _serial_point_cloud(vol, thr, xc=xc))
