@@ -170,6 +170,82 @@ int ct_hip_set_filter_backend(ct_hip_ctx* ctx, int backend);
 int ct_hip_filter_sinograms(ct_hip_ctx* ctx, const float* sinos, int nz, int na, int bins,
                             int filter, float* out);
 
+/* ------------------------------------------------------------ batched form */
+
+/* One volume of a batch: the input of reconstructVolumeFromSinograms
+ * (IReconstructionBackend.h:36-41) in device memory, sinograms (nz, na, bins),
+ * its angles in degrees (host array of na), a filter id, and a device output
+ * of (nz, bins, bins). Each volume is reconstructed exactly as
+ * ct_hip_reconstruct_from_sinograms_dev would reconstruct it alone. */
+typedef struct {
+    const float* d_sinos;
+    float* d_out;
+    const float* angles_deg;
+    int nz, na, bins, filter;
+} ct_fbp_job;
+
+enum { CT_BATCH_GRAPH = 1,     /* capture the launches once in a hipGraph and replay it */
+       CT_BATCH_AUTOTUNE = 2   /* pick the backprojection tiles with the autotuner */ };
+
+/* Batched consumer path: the row filters of every volume run in one
+ * persistent launch and the backprojections in a second, both fed from work
+ * lists; there is no reference counterpart (the reference loops per slice,
+ * BackendFactory.cpp:17-60). Waits for completion. */
+int ct_hip_fbp_batch_dev(ct_hip_ctx* ctx, const ct_fbp_job* jobs, int njobs, int flags);
+
+/* Backprojection tile configurations (voxel tile edge, angles per LDS stage)
+ * and the autotuner that picks one per (N, na) by timing them all on the
+ * device (cached in the context and, when CT_HIP_TUNE_CACHE names a file,
+ * there). ms_out (optional, n_ms entries) receives each configuration's time. */
+int ct_hip_bp_config_count(void);
+int ct_hip_bp_config(int index, int* tile, int* angles);
+int ct_hip_autotune(ct_hip_ctx* ctx, int N, int na, int* config_out, double* ms_out, int n_ms);
+/* Configuration used by the per-volume calls (default 0). */
+int ct_hip_set_bp_config(ct_hip_ctx* ctx, int config);
+
+/* ------------------------------------------------------------ serving form */
+
+/* A request loop over device-resident sinograms with continuous batching
+ * (no reference counterpart). The server owns a context and one device pool
+ * of pool_bytes for outputs and filtered-sinogram workspaces; jobs are
+ * admitted FIFO when both fit, batched up to max_batch, and run in up to
+ * `slots` batches in flight on separate streams. Each job's result equals
+ * its standalone reconstructVolumeFromSinograms result bit for bit. */
+typedef struct ct_hip_server ct_hip_server;
+int ct_hip_server_create(int device, size_t pool_bytes, int max_batch, int slots,
+                         ct_hip_server** out);
+void ct_hip_server_destroy(ct_hip_server* srv);
+/* CT_BATCH_* flags for the server's batches (default graph + autotune). */
+int ct_hip_server_set_flags(ct_hip_server* srv, int flags);
+/* job->d_sinos must stay valid until the job is finished; job->d_out is
+ * ignored (the output is placed in the pool). angles are copied. */
+int ct_hip_server_submit(ct_hip_server* srv, const ct_fbp_job* job, uint64_t* ticket);
+/* Waits up to timeout_ms for any finished job: CT_OK with its ticket, device
+ * output and submit-to-completion latency; 1 on timeout; a negative code if
+ * that job failed. */
+int ct_hip_server_wait(ct_hip_server* srv, double timeout_ms, uint64_t* ticket,
+                       const float** d_out, double* latency_ms);
+int ct_hip_server_download(ct_hip_server* srv, uint64_t ticket, float* host_out, size_t bytes);
+/* Returns a finished job's output to the pool. */
+int ct_hip_server_release(ct_hip_server* srv, uint64_t ticket);
+/* [0] batches [1] completed [2] max jobs in flight [3] launches overlapping a
+ * running batch [4] pool bytes in use [5] pool bytes [6] ms first submit ->
+ * last completion */
+int ct_hip_server_stats(ct_hip_server* srv, double* stats, int n);
+const char* ct_hip_server_last_error(const ct_hip_server* srv);
+
+/* ------------------------------------------------- synthetic acquisition */
+
+/* Forward projection alone, device-resident: computeSinogram
+ * (SerialBackend.cpp:59-136) of every slice, d_sino (d, na, bins);
+ * minmax_out (optional host, 2*d) receives {min,max} per slice. */
+int ct_hip_radon_dev(ct_hip_ctx* ctx, const float* d_vol, int w, int h, int d, int num_angles,
+                     int bins, float* d_sino, float* minmax_out);
+/* In place: counts ~ Poisson(I0 exp(-kappa p)) (normal approximation), then
+ * p = -ln(max(counts,1)/I0)/kappa. Synthetic input for the serving form. */
+int ct_hip_noisy_log_dev(ct_hip_ctx* ctx, float* d_sino, size_t n, float I0, float kappa,
+                         uint64_t seed);
+
 /* Timings of the last volume call, milliseconds, measured with HIP events on
  * the context's stream: [0] normalise+Radon, [1] row filter,
  * [2] backprojection (incl. denormalisation), [3] total.

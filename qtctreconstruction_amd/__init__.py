@@ -57,6 +57,16 @@ class Ellipsoid(ctypes.Structure):
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
+
+
+class FbpJob(ctypes.Structure):
+    """ct_fbp_job (include/ct_hip.h): one volume of a batch, device pointers."""
+    _fields_ = [("d_sinos", ctypes.c_void_p), ("d_out", ctypes.c_void_p),
+                ("angles_deg", ctypes.POINTER(ctypes.c_float)), ("nz", ctypes.c_int),
+                ("na", ctypes.c_int), ("bins", ctypes.c_int), ("filter", ctypes.c_int)]
+
+
+CT_BATCH_GRAPH, CT_BATCH_AUTOTUNE = 1, 2
 _lib = None
 
 
@@ -98,6 +108,23 @@ def load_library(path: str = LIB_PATH):
                                                ctypes.c_float, _f32p, sz, ctypes.POINTER(sz)]),
         "ct_hip_design_filter": (i, [ctx, i, i, _f32p]),
         "ct_hip_set_filter_backend": (i, [ctx, i]),
+        "ct_hip_fbp_batch_dev": (i, [ctx, ctypes.POINTER(FbpJob), i, i]),
+        "ct_hip_bp_config_count": (i, []),
+        "ct_hip_bp_config": (i, [i, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "ct_hip_autotune": (i, [ctx, i, i, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double), i]),
+        "ct_hip_set_bp_config": (i, [ctx, i]),
+        "ct_hip_server_create": (i, [i, sz, i, i, ctypes.POINTER(_vp)]),
+        "ct_hip_server_destroy": (None, [_vp]),
+        "ct_hip_server_set_flags": (i, [_vp, i]),
+        "ct_hip_server_submit": (i, [_vp, ctypes.POINTER(FbpJob), ctypes.POINTER(ctypes.c_uint64)]),
+        "ct_hip_server_wait": (i, [_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64),
+                                   ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_double)]),
+        "ct_hip_server_download": (i, [_vp, ctypes.c_uint64, _f32p, sz]),
+        "ct_hip_server_release": (i, [_vp, ctypes.c_uint64]),
+        "ct_hip_server_stats": (i, [_vp, ctypes.POINTER(ctypes.c_double), i]),
+        "ct_hip_server_last_error": (ctypes.c_char_p, [_vp]),
+        "ct_hip_radon_dev": (i, [ctx, _vp, i, i, i, i, i, _vp, _f32p]),
+        "ct_hip_noisy_log_dev": (i, [ctx, _vp, sz, ctypes.c_float, ctypes.c_float, ctypes.c_uint64]),
         "ct_hip_filter_sinograms": (i, [ctx, _f32p, i, i, i, i, _f32p]),
         "ct_hip_last_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double)]),
     }
@@ -120,6 +147,11 @@ def exported_symbols() -> List[str]:
         "ct_hip_reconstruct_from_sinograms", "ct_hip_reconstruct_from_sinograms_dev",
         "ct_hip_extract_point_cloud", "ct_hip_extract_point_cloud_dev",
         "ct_hip_design_filter", "ct_hip_set_filter_backend", "ct_hip_filter_sinograms",
+        "ct_hip_fbp_batch_dev", "ct_hip_bp_config_count", "ct_hip_bp_config", "ct_hip_autotune",
+        "ct_hip_set_bp_config", "ct_hip_server_create", "ct_hip_server_destroy",
+        "ct_hip_server_set_flags", "ct_hip_server_submit", "ct_hip_server_wait",
+        "ct_hip_server_download", "ct_hip_server_release", "ct_hip_server_stats",
+        "ct_hip_server_last_error", "ct_hip_radon_dev", "ct_hip_noisy_log_dev",
         "ct_hip_last_timings",
     ]
 
@@ -238,6 +270,37 @@ class Context:
         self._call("ct_hip_design_filter", self.handle, n, int(ftype), _p(out))
         return out
 
+    def bp_configs(self):
+        """[(tile, angles_per_stage)] of the instantiated backprojection configurations."""
+        out = []
+        for k in range(self.lib.ct_hip_bp_config_count()):
+            t, a = ctypes.c_int(), ctypes.c_int()
+            self.lib.ct_hip_bp_config(k, ctypes.byref(t), ctypes.byref(a))
+            out.append((t.value, a.value))
+        return out
+
+    def set_bp_config(self, cfg: int):
+        self._call("ct_hip_set_bp_config", self.handle, int(cfg))
+
+    def autotune(self, N: int, na: int):
+        """(best config index, per-config milliseconds)."""
+        n = self.lib.ct_hip_bp_config_count()
+        best = ctypes.c_int()
+        ms = (ctypes.c_double * n)()
+        self._call("ct_hip_autotune", self.handle, N, na, ctypes.byref(best), ms, n)
+        return best.value, list(ms)
+
+    def fbp_batch(self, jobs, flags: int = CT_BATCH_GRAPH | CT_BATCH_AUTOTUNE):
+        """jobs: list of (d_sinos DeviceBuffer, d_out DeviceBuffer, angles, nz, na, bins, filter)."""
+        keep = []
+        arr = (FbpJob * len(jobs))()
+        for k, (ds, do, ang, nz, na, bins, f) in enumerate(jobs):
+            a = _f32(ang)
+            keep.append(a)
+            arr[k] = FbpJob(ds.ptr, do.ptr, _p(a), nz, na, bins, int(f))
+        self._call("ct_hip_fbp_batch_dev", self.handle, arr, len(jobs), int(flags))
+        return self.timings()
+
     def set_filter_backend(self, backend: int):
         self._call("ct_hip_set_filter_backend", self.handle, int(backend))
 
@@ -249,6 +312,16 @@ class Context:
         out = np.empty((nz, na, sq), np.float32)
         self._call("ct_hip_filter_sinograms", self.handle, _p(s), nz, na, bins, int(ftype), _p(out))
         return out
+
+    def radon_dev(self, d_vol: DeviceBuffer, w: int, h: int, d: int, na: int, bins: int,
+                  d_sino: DeviceBuffer) -> np.ndarray:
+        mm = np.empty(2 * d, np.float32)
+        self._call("ct_hip_radon_dev", self.handle, d_vol.ptr, w, h, d, na, bins, d_sino.ptr, _p(mm))
+        return mm.reshape(d, 2)
+
+    def noisy_log_dev(self, d_sino: DeviceBuffer, n: int, I0: float, kappa: float, seed: int):
+        self._call("ct_hip_noisy_log_dev", self.handle, d_sino.ptr, n, float(I0), float(kappa),
+                   ctypes.c_uint64(seed))
 
     def phantom(self, seed: int, n: int, nx: int, ny: int, nz: int) -> np.ndarray:
         e = phantom_params(seed, n)
@@ -262,6 +335,78 @@ class Context:
         buf = buf or self.alloc(4 * nx * ny * nz)
         self._call("ct_hip_phantom_dev", self.handle, e, n, nx, ny, nz, buf.ptr)
         return buf
+
+
+class Server:
+    """Serving form (ct_hip_server_*): continuous batching over device-resident sinograms."""
+
+    def __init__(self, pool_bytes: int, max_batch: int = 16, slots: int = 2, device: int = 0):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.ct_hip_server_create(device, int(pool_bytes), max_batch, slots, ctypes.byref(h))
+        if rc != CT_OK:
+            raise BackendError(rc, "ct_hip_server_create failed")
+        self.handle = h.value
+        self._angles = {}
+
+    def _err(self, rc, what):
+        msg = self.lib.ct_hip_server_last_error(self.handle) or b""
+        raise BackendError(rc, f"{what}: {msg.decode(errors='replace')}")
+
+    def set_flags(self, flags: int):
+        self.lib.ct_hip_server_set_flags(self.handle, int(flags))
+
+    def submit(self, d_sinos: "DeviceBuffer", angles, nz: int, na: int, bins: int, ftype: int) -> int:
+        a = _f32(angles)
+        job = FbpJob(d_sinos.ptr, None, _p(a), nz, na, bins, int(ftype))
+        t = ctypes.c_uint64()
+        rc = self.lib.ct_hip_server_submit(self.handle, ctypes.byref(job), ctypes.byref(t))
+        if rc != CT_OK:
+            self._err(rc, "submit")
+        return t.value
+
+    def wait(self, timeout_ms: float = 1000.0):
+        """(ticket, latency_ms) of the next finished job, or None on timeout."""
+        t = ctypes.c_uint64()
+        p = _vp()
+        lat = ctypes.c_double()
+        rc = self.lib.ct_hip_server_wait(self.handle, timeout_ms, ctypes.byref(t), ctypes.byref(p),
+                                         ctypes.byref(lat))
+        if rc == 1:
+            return None
+        if rc != CT_OK:
+            self._err(rc, f"job {t.value}")
+        return t.value, lat.value
+
+    def download(self, ticket: int, shape) -> np.ndarray:
+        out = np.empty(shape, np.float32)
+        rc = self.lib.ct_hip_server_download(self.handle, ticket, _p(out), out.nbytes)
+        if rc != CT_OK:
+            self._err(rc, "download")
+        return out
+
+    def release(self, ticket: int):
+        rc = self.lib.ct_hip_server_release(self.handle, ticket)
+        if rc != CT_OK:
+            self._err(rc, "release")
+
+    def stats(self) -> dict:
+        v = (ctypes.c_double * 7)()
+        self.lib.ct_hip_server_stats(self.handle, v, 7)
+        keys = ["batches", "completed", "max_inflight_jobs", "overlapped_launches",
+                "pool_used", "pool_size", "span_ms"]
+        return dict(zip(keys, list(v)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.ct_hip_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class HipBackend:

@@ -1,0 +1,348 @@
+// batch.cpp — batched form of reconstructVolumeFromSinograms
+// (IReconstructionBackend.h:36-41): many volumes of uneven size (edge, depth,
+// angle count, filter) reconstructed by two persistent launches, the row
+// filter over all (volume, angle, z-block) items and the backprojection over
+// all (volume, z-block, tile) items, both fed from device work lists through
+// atomic counters. The pair (plus the counter reset) is captured once into a
+// hipGraph and replayed for later batches: only the work lists change, and
+// they are uploaded before the replay, outside the graph.
+//
+// The backprojection tile configuration (voxel tile, angles per LDS stage)
+// comes from the autotuner, which times every instantiated configuration on
+// a synthetic volume of the job's class and caches the winner per (N, na),
+// optionally in a file named by CT_HIP_TUNE_CACHE.
+#include "ctx.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <numeric>
+#include <sstream>
+
+using namespace cthip;
+
+namespace cthost {
+
+// Per-stream batch launcher state. The work-list buffers are sized for
+// kMaxBatch jobs once, so captured graphs (keyed by the launch shape) stay
+// valid across batches; the tables are staged through pinned host memory.
+BatchSlot::~BatchSlot() = default;
+
+void slot_init(ct_hip_ctx* c, BatchSlot* s, bool own_stream) {
+    if (s->ready) return;
+    if (own_stream) {
+        hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "slot stream");
+        s->own_stream = true;
+    } else {
+        s->stream = c->stream;
+    }
+    hip_check(hipEventCreate(&s->ev_start), "slot event");
+    hip_check(hipEventCreate(&s->ev_done), "slot event");
+    s->ftab = static_cast<FiltJob*>(dmalloc(c, sizeof(FiltJob) * kMaxBatch));
+    s->btab = static_cast<BpJob*>(dmalloc(c, sizeof(BpJob) * kMaxBatch));
+    s->hdr = static_cast<int*>(dmalloc(c, sizeof(int) * 4));
+    s->counters = static_cast<int*>(dmalloc(c, sizeof(int) * 2));
+    hip_check(hipHostMalloc((void**)&s->h_ftab, sizeof(FiltJob) * kMaxBatch, hipHostMallocDefault),
+              "pinned table");
+    hip_check(hipHostMalloc((void**)&s->h_btab, sizeof(BpJob) * kMaxBatch, hipHostMallocDefault),
+              "pinned table");
+    hip_check(hipHostMalloc((void**)&s->h_hdr, sizeof(int) * 4, hipHostMallocDefault), "pinned hdr");
+    s->ready = true;
+}
+
+void slot_release(ct_hip_ctx* c, BatchSlot* s) {
+    if (!s->ready) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto& kv : s->graphs) (void)hipGraphExecDestroy(kv.second);
+    s->graphs.clear();
+    dfree(c, s->ftab);
+    dfree(c, s->btab);
+    dfree(c, s->hdr);
+    dfree(c, s->counters);
+    (void)hipHostFree(s->h_ftab);
+    (void)hipHostFree(s->h_btab);
+    (void)hipHostFree(s->h_hdr);
+    (void)hipEventDestroy(s->ev_start);
+    (void)hipEventDestroy(s->ev_done);
+    if (s->own_stream) (void)hipStreamDestroy(s->stream);
+    *s = BatchSlot();
+}
+
+struct BatchGraph {
+    BatchSlot slot;
+    Buf filt;
+};
+
+void batch_release(ct_hip_ctx* c) {
+    if (!c->batch) return;
+    slot_release(c, &c->batch->slot);
+    release(c, c->batch->filt);
+    delete c->batch;
+    c->batch = nullptr;
+}
+
+static void load_tune_cache(ct_hip_ctx* c) {
+    static bool loaded = false;
+    if (loaded) return;
+    loaded = true;
+    const char* path = std::getenv("CT_HIP_TUNE_CACHE");
+    if (!path) return;
+    std::ifstream f(path);
+    int N, na, cfg;
+    while (f >> N >> na >> cfg)
+        if (cfg >= 0 && cfg < kNumBpConfigs) c->tuned[{N, na}] = cfg;
+}
+
+static void store_tune_cache(int N, int na, int cfg) {
+    const char* path = std::getenv("CT_HIP_TUNE_CACHE");
+    if (!path) return;
+    std::ofstream f(path, std::ios::app);
+    f << N << " " << na << " " << cfg << "\n";
+}
+
+// Times every configuration on a synthetic volume of N x N, na angles, deep
+// enough for about 2048 tiles (the chip holds 256 CUs).
+int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times) {
+    load_tune_cache(c);
+    auto it = c->tuned.find({N, na});
+    if (it != c->tuned.end() && !times) return it->second;
+    const Geom g = geometry(N);
+    const int tiles16 = ((N + 15) / 16) * ((N + 15) / 16);
+    int nzb = std::max(1, 2048 * 4 / std::max(1, tiles16));
+    const size_t per_zb = (size_t)na * g.sq * sizeof(float4) + sizeof(float) * kZB * (size_t)N * N;
+    nzb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nzb, (size_t)(1u << 30) / per_zb));
+    const int nz = nzb * kZB;
+    float4* filt = static_cast<float4*>(dmalloc(c, (size_t)nzb * na * g.sq * sizeof(float4)));
+    float* out = static_cast<float*>(dmalloc(c, sizeof(float) * (size_t)nz * N * N));
+    std::vector<double> ms(kNumBpConfigs, 1e30);
+    try {
+        hip_check(hipMemsetAsync(filt, 0, (size_t)nzb * na * g.sq * sizeof(float4), c->stream), "zero");
+        std::vector<float> ang(na);
+        for (int a = 0; a < na; ++a) ang[a] = (180.0f / (float)na) * (float)a;
+        const ct_hip_ctx::Trig t = get_trig(c, ang.data(), na);
+        BpArgs p;
+        p.filt = filt;
+        p.out = out;
+        p.cos_t = t.cos;
+        p.sin_t = t.sin;
+        p.mm = nullptr;
+        p.nz = nz;
+        p.na = na;
+        p.sq = g.sq;
+        p.N = N;
+        p.scale = 1.0f;
+        p.check = bp_needs_check(N, g.sq) ? 1 : 0;
+        for (int cfg = 0; cfg < kNumBpConfigs; ++cfg) {
+            hip_check(launch_backproject_cfg(cfg, p, c->stream), "autotune warmup");
+            for (int r = 0; r < 3; ++r) {
+                hip_check(hipEventRecord(c->ev[0], c->stream), "event");
+                hip_check(launch_backproject_cfg(cfg, p, c->stream), "autotune");
+                hip_check(hipEventRecord(c->ev[1], c->stream), "event");
+                hip_check(hipEventSynchronize(c->ev[1]), "event sync");
+                float e = 0;
+                hip_check(hipEventElapsedTime(&e, c->ev[0], c->ev[1]), "elapsed");
+                ms[cfg] = std::min(ms[cfg], (double)e);
+            }
+        }
+    } catch (...) {
+        dfree(c, filt);
+        dfree(c, out);
+        throw;
+    }
+    dfree(c, filt);
+    dfree(c, out);
+    const int best = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    c->tuned[{N, na}] = best;
+    store_tune_cache(N, na, best);
+    if (times) *times = ms;
+    return best;
+}
+
+size_t filt_elems(const ct_fbp_job& j) {
+    const Geom g = geometry(j.bins);
+    return (size_t)((j.nz + kZB - 1) / kZB) * j.na * g.sq;
+}
+
+size_t job_filt_bytes(const ct_fbp_job& j) { return sizeof(float4) * filt_elems(j); }
+
+// Builds the two work lists for jobs[0..nj) (filtered sinograms go to
+// filts[i]) and enqueues counter reset + persistent filter + persistent
+// backprojection on the slot's stream, as a graph replay when flags ask for
+// it. Does not wait. The caller holds c->mu.
+void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, float4* const* filts,
+                   int flags) {
+    need(jobs != nullptr && nj > 0, "fbpBatch: no jobs");
+    need(nj <= kMaxBatch, "fbpBatch: too many jobs in one batch");
+    int max_nfft = 0;
+    long long work_best = -1;
+    int dom = 0;
+    for (int i = 0; i < nj; ++i) {
+        const ct_fbp_job& j = jobs[i];
+        need(j.d_sinos && j.d_out && j.angles_deg, "fbpBatch: null pointer in job");
+        need(j.nz > 0 && j.na > 0 && j.bins > 0, "fbpBatch: empty job");
+        const Geom g = geometry(j.bins);
+        check_fft(g);
+        max_nfft = std::max(max_nfft, g.nfft);
+        const long long work = (long long)j.bins * j.bins * j.nz * j.na;
+        if (work > work_best) {
+            work_best = work;
+            dom = i;
+        }
+    }
+    const int cfg = (flags & CT_BATCH_AUTOTUNE) ? autotune(c, jobs[dom].bins, jobs[dom].na, nullptr)
+                                                : c->bp_config;
+    const int T = kBpConfigs[cfg].tile;
+    std::vector<int> tiles(nj);
+    int fitems = 0;
+    std::vector<BpJob> bj(nj);
+    for (int i = 0; i < nj; ++i) {
+        const ct_fbp_job& j = jobs[i];
+        const Geom g = geometry(j.bins);
+        const ct_hip_ctx::Trig t = get_trig(c, j.angles_deg, j.na);
+        float* H = get_filter(c, g.nfft, j.filter);
+        float2* tw = get_twiddles(c, g.nfft);
+        s->h_ftab[i].a = make_filt_args(j.d_sinos, j.nz, j.na, j.bins, g.sq, g.pad, g.nfft, H, tw,
+                                        filts[i]);
+        s->h_ftab[i].item_begin = fitems;
+        fitems += ((j.nz + kZB - 1) / kZB) * j.na;
+        BpArgs& p = bj[i].a;
+        p.filt = filts[i];
+        p.out = j.d_out;
+        p.cos_t = t.cos;
+        p.sin_t = t.sin;
+        p.mm = nullptr;
+        p.nz = j.nz;
+        p.na = j.na;
+        p.sq = g.sq;
+        p.N = j.bins;
+        p.scale = 3.14159265358979323846f / (2.0f * (float)j.na);
+        p.check = bp_needs_check(j.bins, g.sq) ? 1 : 0;
+        tiles[i] = ((j.bins + T - 1) / T) * ((j.bins + T - 1) / T);
+    }
+    // filter taps / twiddles / trig tables are made on the context stream
+    if (s->stream != c->stream) hip_check(hipStreamSynchronize(c->stream), "sync device state");
+    // costlier items (more angles) first, so the tail of the launch is short
+    std::vector<int> order(nj);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return jobs[x].na > jobs[y].na; });
+    int bitems = 0;
+    for (int k = 0; k < nj; ++k) {
+        const int i = order[k];
+        s->h_btab[k] = bj[i];
+        s->h_btab[k].item_begin = bitems;
+        bitems += ((jobs[i].nz + kZB - 1) / kZB) * tiles[i];
+    }
+    s->h_hdr[0] = nj;
+    s->h_hdr[1] = fitems;
+    s->h_hdr[2] = nj;
+    s->h_hdr[3] = bitems;
+    hip_check(hipMemcpyAsync(s->ftab, s->h_ftab, sizeof(FiltJob) * nj, hipMemcpyHostToDevice,
+                             s->stream),
+              "upload filter work list");
+    hip_check(hipMemcpyAsync(s->btab, s->h_btab, sizeof(BpJob) * nj, hipMemcpyHostToDevice,
+                             s->stream),
+              "upload backprojection work list");
+    hip_check(hipMemcpyAsync(s->hdr, s->h_hdr, sizeof(int) * 4, hipMemcpyHostToDevice, s->stream),
+              "upload header");
+    const int fgrid = filter_persistent_grid(max_nfft, c->device);
+    const int bgrid = bp_persistent_grid(cfg, c->device);
+    need(fgrid > 0 && bgrid > 0, "fbpBatch: no occupancy for the persistent kernels");
+    auto enqueue = [&]() {
+        hip_check(launch_zero_ints(s->counters, 2, s->stream), "zero counters");
+        hip_check(launch_row_filter_persistent(s->ftab, s->hdr, s->counters, max_nfft, fgrid, s->stream),
+                  "persistent row filter");
+        hip_check(launch_bp_persistent(cfg, s->btab, s->hdr + 2, s->counters + 1, bgrid, s->stream),
+                  "persistent backprojection");
+    };
+    hip_check(hipEventRecord(s->ev_start, s->stream), "event");
+    if (flags & CT_BATCH_GRAPH) {
+        const auto key = std::make_tuple(max_nfft, cfg, fgrid, bgrid);
+        auto it = s->graphs.find(key);
+        if (it == s->graphs.end()) {
+            hipGraph_t graph = nullptr;
+            hip_check(hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed), "begin capture");
+            try {
+                enqueue();
+            } catch (...) {
+                (void)hipStreamEndCapture(s->stream, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(s->stream, &graph), "end capture");
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            hip_check(e, "instantiate");
+            it = s->graphs.emplace(key, exec).first;
+        }
+        hip_check(hipGraphLaunch(it->second, s->stream), "graph launch");
+    } else {
+        enqueue();
+    }
+    hip_check(hipEventRecord(s->ev_done, s->stream), "event");
+}
+
+static void run_batch(ct_hip_ctx* c, const ct_fbp_job* jobs, int nj, int flags) {
+    need(jobs != nullptr && nj > 0, "fbpBatch: no jobs");
+    if (!c->batch) c->batch = new BatchGraph();
+    BatchGraph* b = c->batch;
+    slot_init(c, &b->slot, false);
+    size_t total = 0;
+    std::vector<size_t> off(nj);
+    for (int i = 0; i < nj; ++i) {
+        need(jobs[i].nz > 0 && jobs[i].na > 0 && jobs[i].bins > 0, "fbpBatch: empty job");
+        off[i] = total;
+        total += filt_elems(jobs[i]);
+    }
+    float4* filt = ensure<float4>(c, b->filt, sizeof(float4) * std::max<size_t>(total, 1));
+    std::vector<float4*> f(nj);
+    for (int i = 0; i < nj; ++i) f[i] = filt + off[i];
+    enqueue_batch(c, &b->slot, jobs, nj, f.data(), flags);
+    hip_check(hipEventSynchronize(b->slot.ev_done), "event sync");
+    float t = 0;
+    hip_check(hipEventElapsedTime(&t, b->slot.ev_start, b->slot.ev_done), "elapsed");
+    c->timings[0] = 0;
+    c->timings[1] = 0;
+    c->timings[2] = 0;
+    c->timings[3] = t;
+}
+
+}  // namespace cthost
+
+using namespace cthost;
+
+extern "C" {
+
+int ct_hip_fbp_batch_dev(ct_hip_ctx* c, const ct_fbp_job* jobs, int njobs, int flags) {
+    return guarded(c, [&] { run_batch(c, jobs, njobs, flags); });
+}
+
+int ct_hip_autotune(ct_hip_ctx* c, int N, int na, int* config_out, double* ms_out, int n_ms) {
+    return guarded(c, [&] {
+        need(N > 0 && na > 0, "autotune: bad shape");
+        std::vector<double> t;
+        const int best = autotune(c, N, na, ms_out ? &t : nullptr);
+        if (config_out) *config_out = best;
+        if (ms_out)
+            for (int i = 0; i < n_ms && i < (int)t.size(); ++i) ms_out[i] = t[i];
+    });
+}
+
+int ct_hip_bp_config_count(void) { return kNumBpConfigs; }
+
+int ct_hip_bp_config(int i, int* tile, int* angles) {
+    if (i < 0 || i >= kNumBpConfigs || !tile || !angles) return CT_ERR_INVALID;
+    *tile = kBpConfigs[i].tile;
+    *angles = kBpConfigs[i].angles;
+    return CT_OK;
+}
+
+int ct_hip_set_bp_config(ct_hip_ctx* c, int cfg) {
+    return guarded(c, [&] {
+        need(cfg >= 0 && cfg < kNumBpConfigs, "setBpConfig: unknown configuration");
+        c->bp_config = cfg;
+    });
+}
+
+}  // extern "C"

@@ -146,12 +146,4 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
         }
 }
 
-// Every voxel inside the circle |(xx,yy)| <= R has |u - sq/2| <= R(1+1e-6),
-// so when [sq/2 - R - 1, sq/2 + R + 1] lies in [0, sq-2] no bounds test is
-// needed (always true for the reference geometry output_size == bins).
-inline bool bp_needs_check(int N, int sq) {
-    const double R = (double)(N / 2), dcb = (double)(sq / 2);
-    return !((dcb - R - 1.0 >= 0.0) && (dcb + R + 1.0 <= (double)sq - 2.0));
-}
-
 }  // namespace cthip

@@ -2,8 +2,7 @@
 // context, HBM-budgeted device allocations, cached device state (trig tables,
 // filter banks, twiddles, workspaces) and the stage orchestration of the
 // reference's backend calls (IReconstructionBackend.h:10-55).
-#include "../../include/ct_hip.h"
-#include "ct_internal.h"
+#include "ctx.h"
 
 #include <cmath>
 #include <cstdio>
@@ -19,54 +18,9 @@
 #include <vector>
 
 using namespace cthip;
+using namespace cthost;
 
-namespace {
-
-struct Buf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-struct StatusError : std::runtime_error {
-    int code;
-    StatusError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-}  // namespace
-
-struct ct_hip_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    size_t budget = 0;  // 0 = unlimited
-    size_t in_use = 0;
-    std::recursive_mutex mu;
-    std::string err;
-    std::unordered_map<void*, size_t> sizes;  // every live device allocation
-    // workspaces (grow-only, reused across calls)
-    Buf vol, row, col, sino, filt, out, keys, mm, ell, pc_cnt, pc_pts, pc_coord;
-    // cached device state
-    std::map<std::pair<int, int>, float*> filters;  // (nfft, type) -> H[nfft/2+1]
-    std::map<int, float2*> twiddles;                // nfft -> tw[nfft/2]
-    struct Trig {
-        float* cos = nullptr;
-        float* sin = nullptr;
-    };
-    std::map<std::vector<uint32_t>, Trig> trig;  // keyed by the angle list's bit pattern
-    Trig cur;                                     // table of the current plain call
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    double timings[4] = {0, 0, 0, 0};
-    int filter_backend = 0;  // CT_FILTER_BACKEND_*
-    struct Toep {
-        int Np = 0, Kp = 0;
-        unsigned short *hi = nullptr, *lo = nullptr;
-        unsigned char* f8 = nullptr;
-        float* tmax = nullptr;
-    };
-    std::map<std::tuple<int, int, int>, Toep> toeplitz;  // (nfft, type, bins)
-    Buf rmax;
-};
-
-namespace {
+namespace cthost {
 
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess)
@@ -98,21 +52,6 @@ void dfree(ct_hip_ctx* c, void* p) {
         c->sizes.erase(it);
     }
     (void)hipFree(p);
-}
-
-template <typename T>
-T* ensure(ct_hip_ctx* c, Buf& b, size_t bytes) {
-    if (b.cap < bytes) {
-        if (b.p) {
-            hip_check(hipStreamSynchronize(c->stream), "sync before realloc");
-            dfree(c, b.p);
-            b.p = nullptr;
-            b.cap = 0;
-        }
-        b.p = dmalloc(c, bytes);
-        b.cap = bytes;
-    }
-    return static_cast<T*>(b.p);
 }
 
 void release(ct_hip_ctx* c, Buf& b) {
@@ -179,10 +118,6 @@ float2* get_twiddles(ct_hip_ctx* c, int nfft) {
     c->twiddles[nfft] = t;
     return t;
 }
-
-struct Geom {
-    int bins, sq, pad, nfft;
-};
 
 // SerialBackend.cpp:152-161
 Geom geometry(int bins) {
@@ -279,7 +214,8 @@ void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int
     stage_filter(c, d_sino, nz, na, bins, filter, filt);
     hip_check(hipEventRecord(c->ev[2], c->stream), "event");
     const float scale = 3.14159265358979323846f / (2.0f * (float)na);  // SerialBackend.cpp:246
-    hip_check(launch_backproject(filt, nz, na, g.sq, N, c->cur.cos, c->cur.sin, scale, d_mm, d_out, c->stream),
+    hip_check(launch_backproject(filt, nz, na, g.sq, N, c->cur.cos, c->cur.sin, scale, d_mm, d_out, c->stream,
+                                 c->bp_config),
               "backprojection");
     hip_check(hipEventRecord(c->ev[3], c->stream), "event");
 }
@@ -297,27 +233,6 @@ void read_timings(ct_hip_ctx* c, bool with_radon) {
 }
 
 void bind_device(ct_hip_ctx* c) { hip_check(hipSetDevice(c->device), "hipSetDevice"); }
-
-template <typename F>
-int guarded(ct_hip_ctx* c, F&& f) {
-    if (!c) return CT_ERR_INVALID;
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
-    try {
-        bind_device(c);
-        f();
-        c->err.clear();
-        return CT_OK;
-    } catch (const StatusError& e) {
-        c->err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        c->err = "host allocation failed";
-        return CT_ERR_NOMEM;
-    } catch (const std::exception& e) {
-        c->err = e.what();
-        return CT_ERR_INVALID;
-    }
-}
 
 void need(bool ok, const char* msg) {
     if (!ok) throw StatusError(CT_ERR_INVALID, msg);
@@ -433,6 +348,10 @@ void ct_hip_destroy(ct_hip_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    try {
+        cthost::batch_release(c);
+    } catch (...) {
+    }
     for (auto& kv : c->sizes) (void)hipFree(kv.first);
     c->sizes.clear();
     for (auto& e : c->ev)
@@ -466,6 +385,7 @@ int ct_hip_clear_workspace(ct_hip_ctx* c) {
         }
         c->toeplitz.clear();
         release(c, c->rmax);
+        batch_release(c);
         for (auto& kv : c->twiddles) dfree(c, kv.second);
         c->filters.clear();
         c->twiddles.clear();
@@ -543,6 +463,32 @@ int ct_hip_phantom(ct_hip_ctx* c, const ct_ellipsoid* e, int n, int nx, int ny, 
         float* d = ensure<float>(c, c->vol, bytes);
         phantom_to_device(c, e, n, nx, ny, nz, d);
         hip_check(hipMemcpy(host_out, d, bytes, hipMemcpyDeviceToHost), "download phantom");
+    });
+}
+
+int ct_hip_noisy_log_dev(ct_hip_ctx* c, float* d_sino, size_t n, float I0, float kappa,
+                         uint64_t seed) {
+    return guarded(c, [&] {
+        need(d_sino && n > 0 && I0 > 0.0f && kappa > 0.0f, "noisyLog: bad arguments");
+        hip_check(launch_noisy_log(d_sino, n, I0, kappa, seed, c->stream), "noisy log");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    });
+}
+
+int ct_hip_radon_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, int na, int bins,
+                     float* d_sino, float* minmax_out) {
+    return guarded(c, [&] {
+        need(d_vol && d_sino, "radon: null pointer");
+        need(w > 0 && h > 0 && d > 0 && na > 0 && bins > 0, "radon: empty input");
+        const std::vector<float> ang = serial_angles(na);
+        set_trig(c, ang.data(), na);
+        float* d_mm = ensure<float>(c, c->mm, sizeof(float) * 2 * (size_t)d);
+        stage_radon(c, d_vol, w, h, d, na, bins, d_sino, d_mm);
+        if (minmax_out)
+            hip_check(hipMemcpyAsync(minmax_out, d_mm, sizeof(float) * 2 * (size_t)d,
+                                     hipMemcpyDeviceToHost, c->stream),
+                      "download min/max");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
     });
 }
 

@@ -33,6 +33,8 @@ struct PhEll {
 
 hipError_t launch_phantom(const PhEll* d_ell, int n, int nx, int ny, int nz, float* d_vol,
                           hipStream_t s);
+hipError_t launch_noisy_log(float* d_p, size_t n, float I0, float kappa, unsigned long long seed,
+                            hipStream_t s);
 
 // ---- normalisation + Radon ----------------------------------------------
 // keys: 2*nz uint32 ordered-float keys {min, max} per slice.
@@ -65,6 +67,7 @@ struct FiltJob {
 FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, int pad, int nfft,
                         const float* d_H, const float2* d_tw, float4* d_filt);
 size_t fft_lds_bytes(int nfft);
+int filter_persistent_grid(int max_nfft, int device);
 // hdr = {njobs, total items}; counter must be 0 at launch
 hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
                                         int max_nfft, int grid, hipStream_t s);
@@ -116,6 +119,14 @@ hipError_t launch_backproject(const float4* d_filt, int nz, int na, int sq, int 
                               const float* d_cos, const float* d_sin, float scale,
                               const float* d_mm, float* d_out, hipStream_t s, int cfg = 0);
 hipError_t launch_backproject_cfg(int cfg, const BpArgs& p, hipStream_t s);
+// Every voxel inside the circle |(xx,yy)| <= R has |u - sq/2| <= R(1+1e-6),
+// so when [sq/2 - R - 1, sq/2 + R + 1] lies in [0, sq-2] no bounds test is
+// needed (always true for the reference geometry output_size == bins).
+inline bool bp_needs_check(int N, int sq) {
+    const double R = (double)(N / 2), dcb = (double)(sq / 2);
+    return !((dcb - R - 1.0 >= 0.0) && (dcb + R + 1.0 <= (double)sq - 2.0));
+}
+
 int bp_persistent_grid(int cfg, int device);
 hipError_t launch_bp_persistent(int cfg, const BpJob* d_jobs, const int* d_hdr, int* d_counter,
                                 int grid, hipStream_t s);

@@ -186,6 +186,17 @@ static hipError_t set_lds_attr(const void* fn) {
                                (int)fft_lds_bytes(kMaxFFT));
 }
 
+int filter_persistent_grid(int max_nfft, int device) {
+    int per_cu = 0, cus = 0;
+    if (set_lds_attr((const void*)row_filter_persistent_kernel) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)row_filter_persistent_kernel,
+                                                     256, fft_lds_bytes(max_nfft)) != hipSuccess)
+        return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return 0;
+    return (per_cu < 1 ? 1 : per_cu) * cus;
+}
+
 hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
                                         int max_nfft, int grid, hipStream_t s) {
     static bool attr = false;

@@ -54,6 +54,38 @@ __global__ __launch_bounds__(256) void phantom_kernel(const PhEll* __restrict__ 
     if (x < nx && y < ny) vol[((size_t)z * ny + y) * nx + x] = v;
 }
 
+// Synthetic acquisition of a sinogram p (in place): counts ~ Poisson(I0 e^{-kappa p}),
+// then p~ = -ln(max(counts,1)/I0)/kappa. The Poisson draw uses its normal
+// approximation round(lam + sqrt(lam) z), z from Box-Muller on a counter-based
+// hash of (seed, index); with kappa chosen so that kappa*p <= ~2 the mean is
+// >= I0 e^-2 (1353 at I0 = 1e4), where the approximation is tight.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ void noisy_log_kernel(float* __restrict__ p, size_t n, float I0, float kappa,
+                                 unsigned long long seed) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long h = mix64(seed ^ mix64((unsigned long long)i));
+    const float u1 = ((float)(h >> 40) + 0.5f) * (1.0f / 16777216.0f);
+    const float u2 = (float)((h >> 16) & 0xffffff) * (1.0f / 16777216.0f);
+    const float z = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+    const float lam = I0 * expf(-kappa * p[i]);
+    const float cnt = fmaxf(rintf(lam + sqrtf(lam) * z), 1.0f);
+    p[i] = -logf(cnt / I0) / kappa;
+}
+
+hipError_t launch_noisy_log(float* d_p, size_t n, float I0, float kappa, unsigned long long seed,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(noisy_log_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_p, n,
+                       I0, kappa, seed);
+    return hipGetLastError();
+}
+
 hipError_t launch_phantom(const PhEll* d_ell, int n, int nx, int ny, int nz, float* d_vol,
                           hipStream_t s) {
     dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
