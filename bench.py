@@ -57,6 +57,172 @@ def cpu_baseline(vol_host: np.ndarray, na: int, ftype: int, target_s: float):
                       f"{dt:.2f} s wall"}
 
 
+def precision_form(ctx, d_sino, n, na, sample_z, host_sinos_sample):
+    """Row filter variants on the full C2 sinogram stack: filter-stage time and
+    error vs the exact FP64 padded-Toeplitz filter on sampled slices."""
+    import qtctreconstruction_amd as ct
+    from oracle import oracle as O
+    sq, pad, nfft = O.geometry(n)
+    res = {}
+    d_out = ctx.alloc(4 * n * n * n)
+    ang = O.serial_angles(na)
+    for fname, ftype in (("Ramp", 0), ("Hann", 4)):
+        h = O.kernel64(nfft, ftype)
+        i = np.arange(sq)[:, None]
+        j = np.arange(n)[None, :] + pad
+        T = h[(i - j) % nfft]
+        ref = (host_sinos_sample.astype(np.float64).reshape(-1, n) @ T.T)
+        for fb in ct.FilterBackend:
+            ctx.set_filter_backend(fb)
+            # whole-stack timing through the consumer path (filter + backprojection)
+            for _ in range(2):
+                ctx._call("ct_hip_reconstruct_from_sinograms_dev", ctx.handle, d_sino.ptr, n, na, n,
+                          ang.ctypes.data_as(__import__("ctypes").POINTER(__import__("ctypes").c_float)),
+                          ftype, d_out.ptr)
+            t = ctx.timings()
+            got = ctx.filter_sinograms(host_sinos_sample, ftype).reshape(ref.shape)
+            res[f"{fname}/{fb.name}"] = {"filter_ms": float(t[1]), "backprojection_ms": float(t[2]),
+                                         "rel_l2_vs_fp64": O.rel_l2(got, ref),
+                                         "max_abs_vs_fp64": float(np.abs(got - ref).max())}
+            log(f"[bench] precision {fname}/{fb.name}: {res[f'{fname}/{fb.name}']}")
+    ctx.set_filter_backend(ct.FilterBackend.FFT_F32)
+    d_out.free()
+    flops = 2.0 * n * na * n * sq
+    res["gemm_flop"] = flops
+    res["rows"] = n * na
+    return res
+
+
+def make_volume_sinos(ctx, seed, N, nz, na, n_ell=16):
+    """Device sinograms (nz, na, N) of a seeded phantom, Serial semantics."""
+    d_vol = ctx.alloc(4 * N * N * nz)
+    ctx.phantom_dev(seed, n_ell, N, N, nz, d_vol)
+    d_s = ctx.alloc(4 * nz * na * N)
+    ctx.radon_dev(d_vol, N, N, nz, na, N, d_s)
+    d_vol.free()
+    return d_s
+
+
+def batched_form(ctx, seed=4):
+    """16 uneven volumes, one persistent filter + one persistent backprojection
+    launch (hipGraph, autotuned tiles) vs the same volumes one by one."""
+    import qtctreconstruction_amd as ct
+    rng = np.random.default_rng(seed)
+    shapes = [(128, 128), (256, 256), (384, 192), (512, 256), (768, 128)]
+    jobs = []
+    for k in range(16):
+        N, nz = shapes[rng.integers(len(shapes))]
+        na = int(rng.choice([180, 360, 720, 1440]))
+        f = int(rng.choice([0, 1, 4]))
+        d_s = make_volume_sinos(ctx, 1000 + k, N, nz, na, int(rng.integers(10, 25)))
+        d_o = ctx.alloc(4 * N * N * nz)
+        step = np.float32(180.0) / np.float32(na)
+        ang = (step * np.arange(na, dtype=np.float32)).astype(np.float32)
+        jobs.append((d_s, d_o, ang, nz, na, N, f))
+    updates = float(sum(N * N * nz * na for (_, _, _, nz, na, N, _) in jobs))
+    flags = ct.CT_BATCH_GRAPH | ct.CT_BATCH_AUTOTUNE
+    ctx.fbp_batch(jobs, flags)  # tunes, captures
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.fbp_batch(jobs, flags)
+    t_batch = (time.perf_counter() - t0) / reps
+    import ctypes
+    t0 = time.perf_counter()
+    for (d_s, d_o, ang, nz, na, N, f) in jobs:
+        ctx._call("ct_hip_reconstruct_from_sinograms_dev", ctx.handle, d_s.ptr, nz, na, N,
+                  ang.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), f, d_o.ptr)
+    t_seq = time.perf_counter() - t0
+    for (d_s, d_o, *_r) in jobs:
+        d_s.free()
+        d_o.free()
+    return {"volumes": [[N, N, nz, na] for (_, _, _, nz, na, N, _) in jobs],
+            "voxel_updates": updates, "batched_ms": t_batch * 1e3,
+            "batched_voxel_updates_per_s": updates / t_batch,
+            "one_by_one_ms": t_seq * 1e3, "one_by_one_voxel_updates_per_s": updates / t_seq}
+
+
+def served_form(ctx, n_jobs, seed=5, pool_gb=96.0, rate_factor=0.7):
+    """Poisson stream of jobs over a device-resident input pool; p50/p99
+    latency and throughput, beside plain per-job HIP FBP on the same mix."""
+    import ctypes
+
+    import qtctreconstruction_amd as ct
+    rng = np.random.default_rng(seed)
+    edges = [128, 256, 512, 768, 1024]
+    angle_sets = [360, 720, 1440]
+    I0 = 1e4
+    # one noisy sinogram stack of depth 512 per (edge, angles) class; a job of
+    # depth d uses its first d slices
+    pool = {}
+    log("[bench] serving: generating the input pool")
+    for N in edges:
+        for na in angle_sets:
+            d_s = make_volume_sinos(ctx, 7000 + N + na, N, 512, na, 24)
+            ctx.noisy_log_dev(d_s, 512 * na * N, I0, 2.0 / N, seed=N * 10007 + na)
+            pool[(N, na)] = d_s
+    angles = {na: ((np.float32(180.0) / np.float32(na)) * np.arange(na, dtype=np.float32)).astype(np.float32)
+              for na in angle_sets}
+    jobs = [(int(rng.choice(edges)), int(rng.integers(64, 513)), int(rng.choice(angle_sets)),
+             int(rng.choice([0, 1, 4]))) for _ in range(n_jobs)]
+    upd = np.array([N * N * d * na for (N, d, na, f) in jobs], np.float64)
+    # plain per-job HIP FBP on a sample of the mix -> jobs/s capacity, rate
+    d_out = ctx.alloc(4 * 1024 * 1024 * 512)
+    sample = jobs[: min(40, n_jobs)]
+    t0 = time.perf_counter()
+    for (N, d, na, f) in sample:
+        ctx._call("ct_hip_reconstruct_from_sinograms_dev", ctx.handle, pool[(N, na)].ptr, d, na, N,
+                  angles[na].ctypes.data_as(ctypes.POINTER(ctypes.c_float)), f, d_out.ptr)
+    t_plain = time.perf_counter() - t0
+    plain_ups = float(upd[: len(sample)].sum()) / t_plain
+    d_out.free()
+    mean_job_s = t_plain / len(sample)
+    lam = rate_factor / mean_job_s
+    log(f"[bench] serving: plain {plain_ups:.3e} upd/s, {1/mean_job_s:.1f} jobs/s -> rate {lam:.1f}/s")
+
+    def run(open_loop):
+        srv = ct.Server(pool_bytes=int(pool_gb * (1 << 30)), max_batch=32, slots=2)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs)) if open_loop else np.zeros(n_jobs)
+        lat = []
+        done = 0
+        start = time.perf_counter()
+        k = 0
+        while done < n_jobs:
+            now = time.perf_counter() - start
+            while k < n_jobs and arrivals[k] <= now:
+                N, d, na, f = jobs[k]
+                srv.submit(pool[(N, na)], angles[na], d, na, N, f)
+                k += 1
+            wait_ms = 0.2 if k < n_jobs else 1000.0
+            r = srv.wait(wait_ms)
+            while r is not None:
+                lat.append(r[1])
+                srv.release(r[0])
+                done += 1
+                r = srv.wait(0.0) if done < n_jobs else None
+        span = time.perf_counter() - start
+        st = srv.stats()
+        srv.close()
+        lat = np.array(lat)
+        return {"jobs": n_jobs, "span_s": span, "voxel_updates_per_s": float(upd.sum()) / span,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "batches": st["batches"], "max_jobs_in_flight": st["max_inflight_jobs"],
+                "overlapped_launches": st["overlapped_launches"]}
+
+    log("[bench] serving: closed loop (all jobs queued at t=0)")
+    saturated = run(False)
+    log(f"[bench] serving saturated: {saturated}")
+    log("[bench] serving: Poisson arrivals")
+    poisson = run(True)
+    log(f"[bench] serving poisson: {poisson}")
+    for d_s in pool.values():
+        d_s.free()
+    return {"jobs": n_jobs, "rate_per_s": lam, "rate_factor_of_plain": rate_factor,
+            "plain_per_job_voxel_updates_per_s": plain_ups, "saturated": saturated,
+            "poisson": poisson, "latency": "submit -> completion in device memory",
+            "noise": f"counts ~ Poisson(1e4 exp(-kappa p)) (normal approx.), kappa = 2/N, log"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +235,13 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-target-s", type=float, default=15.0)
     ap.add_argument("--no-parity", action="store_true")
+    ap.add_argument("--forms", default="precision,batched,served",
+                    help="extra forms to measure: precision,batched,served (or none)")
+    ap.add_argument("--serve-jobs", type=int, default=2000)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also launch the known-traffic kernels used to scale HBM counters: "
+                         "phantom_kernel with 0 ellipsoids writes exactly 4*n^3 bytes, "
+                         "minmax_kernel reads exactly 4*n^3 bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by "
@@ -83,6 +256,8 @@ def main():
     nvox = n * n * n
     d_vol = ctx.alloc(4 * nvox)
     d_out = ctx.alloc(4 * nvox)
+    if args.calibrate:
+        ctx.phantom_dev(args.seed, 0, n, n, n, d_vol)
     ctx.phantom_dev(args.seed, args.ellipsoids, n, n, n, d_vol)
     lib = ctx.lib
 
@@ -131,6 +306,25 @@ def main():
         log("[bench] cpu baseline")
         cpu = cpu_baseline(vol_host, na, ftype, args.cpu_target_s)
         log(f"[bench] cpu {cpu}")
+
+    forms = [f for f in args.forms.split(",") if f and f != "none"]
+    if "precision" in forms:
+        from oracle import oracle as O
+        zl = [n // 2, n // 2 + 1]
+        d_sino = ctx.alloc(4 * n * na * n)
+        ctx.radon_dev(d_vol, n, n, n, na, n, d_sino)
+        host_sample = np.stack([d_sino.download((n, na, n))[z] for z in zl]) if vol_host is not None \
+            else d_sino.download((n, na, n))[zl]
+        extra["precision_form"] = precision_form(ctx, d_sino, n, na, zl, host_sample)
+        d_sino.free()
+    if "batched" in forms:
+        log("[bench] batched form")
+        extra["batched_form"] = batched_form(ctx)
+        log(f"[bench] batched {extra['batched_form']['batched_voxel_updates_per_s']:.3e} upd/s")
+    if "served" in forms:
+        d_vol.free()
+        d_out.free()
+        extra["served_form"] = served_form(ctx, args.serve_jobs)
 
     line = {
         "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1,

@@ -41,13 +41,15 @@ void slot_init(ct_hip_ctx* c, BatchSlot* s, bool own_stream) {
     hip_check(hipEventCreate(&s->ev_done), "slot event");
     s->ftab = static_cast<FiltJob*>(dmalloc(c, sizeof(FiltJob) * kMaxBatch));
     s->btab = static_cast<BpJob*>(dmalloc(c, sizeof(BpJob) * kMaxBatch));
-    s->hdr = static_cast<int*>(dmalloc(c, sizeof(int) * 4));
-    s->counters = static_cast<int*>(dmalloc(c, sizeof(int) * 2));
+    s->hdr = static_cast<int*>(dmalloc(c, sizeof(int) * (2 + 2 * kMaxFftClasses)));
+    s->counters = static_cast<int*>(dmalloc(c, sizeof(int) * (1 + kMaxFftClasses)));
     hip_check(hipHostMalloc((void**)&s->h_ftab, sizeof(FiltJob) * kMaxBatch, hipHostMallocDefault),
               "pinned table");
     hip_check(hipHostMalloc((void**)&s->h_btab, sizeof(BpJob) * kMaxBatch, hipHostMallocDefault),
               "pinned table");
-    hip_check(hipHostMalloc((void**)&s->h_hdr, sizeof(int) * 4, hipHostMallocDefault), "pinned hdr");
+    hip_check(hipHostMalloc((void**)&s->h_hdr, sizeof(int) * (2 + 2 * kMaxFftClasses),
+                            hipHostMallocDefault),
+              "pinned hdr");
     s->ready = true;
 }
 
@@ -113,8 +115,17 @@ int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times) {
     const size_t per_zb = (size_t)na * g.sq * sizeof(float4) + sizeof(float) * kZB * (size_t)N * N;
     nzb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nzb, (size_t)(1u << 30) / per_zb));
     const int nz = nzb * kZB;
-    float4* filt = static_cast<float4*>(dmalloc(c, (size_t)nzb * na * g.sq * sizeof(float4)));
-    float* out = static_cast<float*>(dmalloc(c, sizeof(float) * (size_t)nz * N * N));
+    float4* filt = nullptr;
+    float* out = nullptr;
+    try {
+        filt = static_cast<float4*>(dmalloc(c, (size_t)nzb * na * g.sq * sizeof(float4)));
+        out = static_cast<float*>(dmalloc(c, sizeof(float) * (size_t)nz * N * N));
+    } catch (const StatusError& e) {
+        // a budgeted context without room to tune keeps the default tiles
+        dfree(c, filt);
+        if (e.code != CT_ERR_BUDGET && e.code != CT_ERR_NOMEM) throw;
+        return c->bp_config;
+    }
     std::vector<double> ms(kNumBpConfigs, 1e30);
     try {
         hip_check(hipMemsetAsync(filt, 0, (size_t)nzb * na * g.sq * sizeof(float4), c->stream), "zero");
@@ -133,18 +144,32 @@ int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times) {
         p.N = N;
         p.scale = 1.0f;
         p.check = bp_needs_check(N, g.sq) ? 1 : 0;
-        for (int cfg = 0; cfg < kNumBpConfigs; ++cfg) {
-            hip_check(launch_backproject_cfg(cfg, p, c->stream), "autotune warmup");
-            for (int r = 0; r < 3; ++r) {
-                hip_check(hipEventRecord(c->ev[0], c->stream), "event");
-                hip_check(launch_backproject_cfg(cfg, p, c->stream), "autotune");
-                hip_check(hipEventRecord(c->ev[1], c->stream), "event");
-                hip_check(hipEventSynchronize(c->ev[1]), "event sync");
-                float e = 0;
-                hip_check(hipEventElapsedTime(&e, c->ev[0], c->ev[1]), "elapsed");
-                ms[cfg] = std::min(ms[cfg], (double)e);
-            }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hip_check(hipEventCreate(&e0), "event");
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            hip_check(hipErrorOutOfMemory, "event");
         }
+        try {
+            for (int cfg = 0; cfg < kNumBpConfigs; ++cfg) {
+                hip_check(launch_backproject_cfg(cfg, p, c->stream), "autotune warmup");
+                for (int r = 0; r < 3; ++r) {
+                    hip_check(hipEventRecord(e0, c->stream), "event");
+                    hip_check(launch_backproject_cfg(cfg, p, c->stream), "autotune");
+                    hip_check(hipEventRecord(e1, c->stream), "event");
+                    hip_check(hipEventSynchronize(e1), "event sync");
+                    float e = 0;
+                    hip_check(hipEventElapsedTime(&e, e0, e1), "elapsed");
+                    ms[cfg] = std::min(ms[cfg], (double)e);
+                }
+            }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     } catch (...) {
         dfree(c, filt);
         dfree(c, out);
@@ -174,7 +199,6 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
                    int flags) {
     need(jobs != nullptr && nj > 0, "fbpBatch: no jobs");
     need(nj <= kMaxBatch, "fbpBatch: too many jobs in one batch");
-    int max_nfft = 0;
     long long work_best = -1;
     int dom = 0;
     for (int i = 0; i < nj; ++i) {
@@ -183,7 +207,6 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
         need(j.nz > 0 && j.na > 0 && j.bins > 0, "fbpBatch: empty job");
         const Geom g = geometry(j.bins);
         check_fft(g);
-        max_nfft = std::max(max_nfft, g.nfft);
         const long long work = (long long)j.bins * j.bins * j.nz * j.na;
         if (work > work_best) {
             work_best = work;
@@ -194,18 +217,15 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
                                                 : c->bp_config;
     const int T = kBpConfigs[cfg].tile;
     std::vector<int> tiles(nj);
-    int fitems = 0;
     std::vector<BpJob> bj(nj);
+    std::vector<FiltJob> fj(nj);
     for (int i = 0; i < nj; ++i) {
         const ct_fbp_job& j = jobs[i];
         const Geom g = geometry(j.bins);
         const ct_hip_ctx::Trig t = get_trig(c, j.angles_deg, j.na);
         float* H = get_filter(c, g.nfft, j.filter);
         float2* tw = get_twiddles(c, g.nfft);
-        s->h_ftab[i].a = make_filt_args(j.d_sinos, j.nz, j.na, j.bins, g.sq, g.pad, g.nfft, H, tw,
-                                        filts[i]);
-        s->h_ftab[i].item_begin = fitems;
-        fitems += ((j.nz + kZB - 1) / kZB) * j.na;
+        fj[i].a = make_filt_args(j.d_sinos, j.nz, j.na, j.bins, g.sq, g.pad, g.nfft, H, tw, filts[i]);
         BpArgs& p = bj[i].a;
         p.filt = filts[i];
         p.out = j.d_out;
@@ -233,31 +253,62 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
         s->h_btab[k].item_begin = bitems;
         bitems += ((jobs[i].nz + kZB - 1) / kZB) * tiles[i];
     }
+    // Filter work lists, one per FFT length: a persistent launch reserves LDS
+    // for its largest transform, so lengths are not mixed in one launch.
+    std::map<int, std::vector<int>> classes;  // nfft -> jobs
+    for (int i = 0; i < nj; ++i) classes[fj[i].a.nfft].push_back(i);
+    need((int)classes.size() <= kMaxFftClasses, "fbpBatch: too many FFT lengths");
+    std::vector<int> cls_nfft, cls_first;
+    int pos = 0, ci = 0;
+    for (auto& kv : classes) {
+        int items = 0;
+        cls_nfft.push_back(kv.first);
+        cls_first.push_back(pos);
+        for (int i : kv.second) {
+            s->h_ftab[pos] = fj[i];
+            s->h_ftab[pos].item_begin = items;
+            items += ((jobs[i].nz + kZB - 1) / kZB) * jobs[i].na;
+            ++pos;
+        }
+        s->h_hdr[2 + 2 * ci] = (int)kv.second.size();
+        s->h_hdr[3 + 2 * ci] = items;
+        ++ci;
+    }
     s->h_hdr[0] = nj;
-    s->h_hdr[1] = fitems;
-    s->h_hdr[2] = nj;
-    s->h_hdr[3] = bitems;
+    s->h_hdr[1] = bitems;
     hip_check(hipMemcpyAsync(s->ftab, s->h_ftab, sizeof(FiltJob) * nj, hipMemcpyHostToDevice,
                              s->stream),
               "upload filter work list");
     hip_check(hipMemcpyAsync(s->btab, s->h_btab, sizeof(BpJob) * nj, hipMemcpyHostToDevice,
                              s->stream),
               "upload backprojection work list");
-    hip_check(hipMemcpyAsync(s->hdr, s->h_hdr, sizeof(int) * 4, hipMemcpyHostToDevice, s->stream),
+    hip_check(hipMemcpyAsync(s->hdr, s->h_hdr, sizeof(int) * (2 + 2 * ci), hipMemcpyHostToDevice,
+                             s->stream),
               "upload header");
-    const int fgrid = filter_persistent_grid(max_nfft, c->device);
+    std::vector<int> fgrid(ci);
+    for (int k = 0; k < ci; ++k) {
+        fgrid[k] = filter_persistent_grid(cls_nfft[k], c->device);
+        need(fgrid[k] > 0, "fbpBatch: no occupancy for the persistent row filter");
+    }
     const int bgrid = bp_persistent_grid(cfg, c->device);
-    need(fgrid > 0 && bgrid > 0, "fbpBatch: no occupancy for the persistent kernels");
+    need(bgrid > 0, "fbpBatch: no occupancy for the persistent backprojection");
     auto enqueue = [&]() {
-        hip_check(launch_zero_ints(s->counters, 2, s->stream), "zero counters");
-        hip_check(launch_row_filter_persistent(s->ftab, s->hdr, s->counters, max_nfft, fgrid, s->stream),
-                  "persistent row filter");
-        hip_check(launch_bp_persistent(cfg, s->btab, s->hdr + 2, s->counters + 1, bgrid, s->stream),
+        hip_check(launch_zero_ints(s->counters, 1 + ci, s->stream), "zero counters");
+        for (int k = 0; k < ci; ++k)
+            hip_check(launch_row_filter_persistent(s->ftab + cls_first[k], s->hdr + 2 + 2 * k,
+                                                   s->counters + 1 + k, cls_nfft[k], fgrid[k],
+                                                   s->stream),
+                      "persistent row filter");
+        hip_check(launch_bp_persistent(cfg, s->btab, s->hdr, s->counters, bgrid, s->stream),
                   "persistent backprojection");
     };
     hip_check(hipEventRecord(s->ev_start, s->stream), "event");
     if (flags & CT_BATCH_GRAPH) {
-        const auto key = std::make_tuple(max_nfft, cfg, fgrid, bgrid);
+        std::vector<int> key{cfg, bgrid};
+        for (int k = 0; k < ci; ++k) {
+            key.push_back(cls_nfft[k]);
+            key.push_back(fgrid[k]);
+        }
         auto it = s->graphs.find(key);
         if (it == s->graphs.end()) {
             hipGraph_t graph = nullptr;
@@ -342,6 +393,7 @@ int ct_hip_set_bp_config(ct_hip_ctx* c, int cfg) {
     return guarded(c, [&] {
         need(cfg >= 0 && cfg < kNumBpConfigs, "setBpConfig: unknown configuration");
         c->bp_config = cfg;
+        c->bp_config_fixed = true;
     });
 }
 

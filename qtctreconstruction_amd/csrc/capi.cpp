@@ -144,8 +144,9 @@ size_t nzb_of(int nz) { return (size_t)((nz + kZB - 1) / kZB); }
 void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na, int bins,
                  float* d_sino, float* d_mm) {
     unsigned* keys = ensure<unsigned>(c, c->keys, sizeof(unsigned) * 2 * (size_t)nz);
-    float4* row = ensure<float4>(c, c->row, sizeof(float4) * nzb_of(nz) * (size_t)w * h);
-    float4* col = ensure<float4>(c, c->col, sizeof(float4) * nzb_of(nz) * (size_t)w * h);
+    const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
+    float4* row = ensure<float4>(c, c->row, sizeof(float4) * 2 * rzb * (size_t)w * h);
+    float4* col = ensure<float4>(c, c->col, sizeof(float4) * 2 * rzb * (size_t)w * h);
     hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
@@ -161,14 +162,14 @@ const ct_hip_ctx::Toep& get_toeplitz(ct_hip_ctx* c, const Geom& g, int type) {
     if (it != c->toeplitz.end()) return it->second;
     ct_hip_ctx::Toep t;
     t.Np = (g.sq + 255) / 256 * 256;
-    t.Kp = (g.bins + 31) / 32 * 32;
+    t.Kp = (g.bins + 1 + 31) / 32 * 32;  // dp has bins+1 entries
     const size_t n = (size_t)t.Np * t.Kp;
     t.hi = static_cast<unsigned short*>(dmalloc(c, 2 * n));
     t.lo = static_cast<unsigned short*>(dmalloc(c, 2 * n));
     t.f8 = static_cast<unsigned char*>(dmalloc(c, n + 8));
     t.tmax = static_cast<float*>(dmalloc(c, sizeof(float)));
     double* H = static_cast<double*>(dmalloc(c, sizeof(double) * (g.nfft / 2 + 1)));
-    double* h = static_cast<double*>(dmalloc(c, sizeof(double) * g.nfft));
+    double* h = static_cast<double*>(dmalloc(c, sizeof(double) * 2 * g.nfft));  // h, then G
     float* T32 = static_cast<float*>(dmalloc(c, sizeof(float) * n));
     hip_check(launch_toeplitz_design(g.nfft, type, g.sq, g.bins, g.pad, t.Np, t.Kp, H, h, t.hi,
                                      t.lo, T32, t.f8, t.tmax, c->stream),
@@ -209,13 +210,14 @@ void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int
                const float* d_mm, float* d_out) {
     const Geom g = geometry(bins);
     if (c->filter_backend == CT_FILTER_BACKEND_FFT_F32) check_fft(g);
+    const int cfg = c->bp_config_fixed ? c->bp_config : autotune(c, N, na, nullptr);
     float4* filt = ensure<float4>(c, c->filt, sizeof(float4) * nzb_of(nz) * (size_t)na * g.sq);
     hip_check(hipEventRecord(c->ev[1], c->stream), "event");
     stage_filter(c, d_sino, nz, na, bins, filter, filt);
     hip_check(hipEventRecord(c->ev[2], c->stream), "event");
     const float scale = 3.14159265358979323846f / (2.0f * (float)na);  // SerialBackend.cpp:246
     hip_check(launch_backproject(filt, nz, na, g.sq, N, c->cur.cos, c->cur.sin, scale, d_mm, d_out, c->stream,
-                                 c->bp_config),
+                                 cfg),
               "backprojection");
     hip_check(hipEventRecord(c->ev[3], c->stream), "event");
 }

@@ -15,6 +15,8 @@ namespace cthip {
 // done once per (voxel|ray, angle) and reused for the 4 slices, and the
 // normalised volume / filtered sinogram are stored z-innermost as float4.
 constexpr int kZB = 4;
+// Slices per Radon z-block (the normalised copies are [zb][..][..][8]).
+constexpr int kRZ = 8;
 
 // Largest FFT length of the LDS row filter (bins <= 1448).
 constexpr int kMaxFFT = 4096;
@@ -44,7 +46,7 @@ hipError_t launch_minmax(const float* d_vol, size_t slice_elems, int nz, unsigne
 // writes mm[2*z] = min, mm[2*z+1] = max as floats
 hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hipStream_t s);
 // w = (raw == 0) ? 0 : (raw - min) * (1/span); z-blocked row layout [zb][h][w]
-// and column layout [zb][w][h] (float4 per pixel = 4 slices).
+// and column layout [zb][w][h], kRZ = 8 slices per pixel (2 float4).
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, hipStream_t s);
 hipError_t launch_radon(const float4* d_row, const float4* d_col, int w, int h, int nz, int na,
@@ -78,8 +80,9 @@ hipError_t launch_row_filter_fft(const float* d_sino, int nz, int na, int bins, 
                                  int nfft, const float* d_H, const float2* d_tw, float4* d_filt,
                                  hipStream_t s);
 
-// Toeplitz-GEMM row filter (MFMA). T is Np x Kp (Np = sq rounded up to 256,
-// Kp = bins rounded up to 32); variant 1 BF16, 2 split BF16, 3 FP8 E4M3.
+// Toeplitz-GEMM row filter (MFMA), taken by parts. T is Np x Kp (Np = sq
+// rounded up to 256, Kp = bins+1 rounded up to 32); d_h holds 2*nfft doubles
+// (kernel, then its cumulative sum); variant 1 BF16, 2 split BF16, 3 FP8 E4M3.
 hipError_t launch_toeplitz_design(int nfft, int type, int sq, int bins, int pad, int Np, int Kp,
                                   double* d_H, double* d_h, unsigned short* d_Thi,
                                   unsigned short* d_Tlo, float* d_T32, unsigned char* d_T8,

@@ -35,6 +35,8 @@ struct BatchGraph;  // batch.cpp
 
 // Largest number of volumes in one batch.
 constexpr int kMaxBatch = 256;
+// Distinct FFT lengths in one batch (64 .. 4096).
+constexpr int kMaxFftClasses = 8;
 
 // Launch state of one stream that runs batches (batch.cpp).
 struct BatchSlot {
@@ -48,7 +50,7 @@ struct BatchSlot {
     cthip::FiltJob* h_ftab = nullptr;  // pinned staging
     cthip::BpJob* h_btab = nullptr;
     int* h_hdr = nullptr;
-    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+    std::map<std::vector<int>, hipGraphExec_t> graphs;  // keyed by the launch shape
     ~BatchSlot();
 };
 
@@ -79,6 +81,7 @@ struct ct_hip_ctx {
     double timings[4] = {0, 0, 0, 0};
     int filter_backend = 0;  // CT_FILTER_BACKEND_*
     int bp_config = 0;       // backprojection tile configuration (kBpConfigs index)
+    bool bp_config_fixed = false;  // set by ct_hip_set_bp_config; else tuned per (N, na)
     struct Toep {
         int Np = 0, Kp = 0;
         unsigned short *hi = nullptr, *lo = nullptr;

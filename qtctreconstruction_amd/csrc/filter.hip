@@ -100,20 +100,44 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
         X1[br] = make_float2(v[2], v[3]);
     }
     __syncthreads();
-    // forward: decimation in time, natural-order output
-    for (int len = 2; len <= n; len <<= 1) {
-        const int half = len >> 1;
-        const int tstep = n / len;
+    // Forward: decimation in time from bit-reversed input, natural-order
+    // output. Radix-2 stages of half-span h are fused in pairs (h, 2h): a
+    // thread takes the quadruple base + {0, h, 2h, 3h} through both stages in
+    // registers, the same operations as two radix-2 stages with half the LDS
+    // round trips. An odd number of stages starts with one radix-2 stage.
+    int h = 1;
+    if (logn & 1) {
         for (int b = tid; b < nh; b += 256) {
-            const int g = b / half, k = b - g * half;
-            const int i0 = g * len + k, i1 = i0 + half;
-            const float2 w = T[k * tstep];
-            const float2 u0 = X0[i0], v0 = cmul(X0[i1], w);
-            const float2 u1 = X1[i0], v1 = cmul(X1[i1], w);
+            const int i0 = 2 * b, i1 = i0 + 1;
+            const float2 u0 = X0[i0], v0 = X0[i1], u1 = X1[i0], v1 = X1[i1];
             X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
             X0[i1] = make_float2(u0.x - v0.x, u0.y - v0.y);
             X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
             X1[i1] = make_float2(u1.x - v1.x, u1.y - v1.y);
+        }
+        __syncthreads();
+        h = 2;
+    }
+    const int nq = n / 4;
+    for (; h < n; h <<= 2) {
+        const int s1 = n / (2 * h), s2 = n / (4 * h);
+        for (int j = tid; j < nq; j += 256) {
+            const int g = j / h, k = j - g * h;
+            const int e0 = g * 4 * h + k, e1 = e0 + h, e2 = e0 + 2 * h, e3 = e0 + 3 * h;
+            const float2 w1 = T[k * s1], w2 = T[k * s2], w3 = T[(k + h) * s2];
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                float2* X = f ? X1 : X0;
+                const float2 x0 = X[e0], x1 = cmul(X[e1], w1), x2 = X[e2], x3 = cmul(X[e3], w1);
+                const float2 a0 = make_float2(x0.x + x1.x, x0.y + x1.y);
+                const float2 a1 = make_float2(x0.x - x1.x, x0.y - x1.y);
+                const float2 a2 = cmul(make_float2(x2.x + x3.x, x2.y + x3.y), w2);
+                const float2 a3 = cmul(make_float2(x2.x - x3.x, x2.y - x3.y), w3);
+                X[e0] = make_float2(a0.x + a2.x, a0.y + a2.y);
+                X[e2] = make_float2(a0.x - a2.x, a0.y - a2.y);
+                X[e1] = make_float2(a1.x + a3.x, a1.y + a3.y);
+                X[e3] = make_float2(a1.x - a3.x, a1.y - a3.y);
+            }
         }
         __syncthreads();
     }
@@ -124,22 +148,43 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
         X1[k] = make_float2(X1[k].x * hv, X1[k].y * hv);
     }
     __syncthreads();
-    // inverse: decimation in frequency, bit-reversed output
-    for (int len = n; len >= 2; len >>= 1) {
-        const int half = len >> 1;
-        const int tstep = n / len;
-        for (int b = tid; b < nh; b += 256) {
-            const int g = b / half, k = b - g * half;
-            const int i0 = g * len + k, i1 = i0 + half;
-            const float2 w = T[k * tstep];
-            const float2 u0 = X0[i0], v0 = X0[i1];
-            const float2 u1 = X1[i0], v1 = X1[i1];
-            X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
-            X0[i1] = cmulc(make_float2(u0.x - v0.x, u0.y - v0.y), w);
-            X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
-            X1[i1] = cmulc(make_float2(u1.x - v1.x, u1.y - v1.y), w);
+    // Inverse: decimation in frequency from natural-order input, bit-reversed
+    // output; stages of half-span 2h and h fused in pairs, then (odd count)
+    // one final radix-2 stage of half-span 1.
+    for (h = n / 4; h >= 1; h >>= 2) {
+        const int s1 = n / (2 * h), s2 = n / (4 * h);
+        for (int j = tid; j < nq; j += 256) {
+            const int g = j / h, k = j - g * h;
+            const int e0 = g * 4 * h + k, e1 = e0 + h, e2 = e0 + 2 * h, e3 = e0 + 3 * h;
+            const float2 w2 = T[k * s2], w3 = T[(k + h) * s2], w1 = T[k * s1];
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                float2* X = f ? X1 : X0;
+                const float2 x0 = X[e0], x1 = X[e1], x2 = X[e2], x3 = X[e3];
+                const float2 a0 = make_float2(x0.x + x2.x, x0.y + x2.y);
+                const float2 a2 = cmulc(make_float2(x0.x - x2.x, x0.y - x2.y), w2);
+                const float2 a1 = make_float2(x1.x + x3.x, x1.y + x3.y);
+                const float2 a3 = cmulc(make_float2(x1.x - x3.x, x1.y - x3.y), w3);
+                X[e0] = make_float2(a0.x + a1.x, a0.y + a1.y);
+                X[e1] = cmulc(make_float2(a0.x - a1.x, a0.y - a1.y), w1);
+                X[e2] = make_float2(a2.x + a3.x, a2.y + a3.y);
+                X[e3] = cmulc(make_float2(a2.x - a3.x, a2.y - a3.y), w1);
+            }
         }
         __syncthreads();
+        if (h == 1) break;
+        if (h == 2) {  // odd number of stages: finish with half-span 1
+            for (int b = tid; b < nh; b += 256) {
+                const int i0 = 2 * b, i1 = i0 + 1;
+                const float2 u0 = X0[i0], v0 = X0[i1], u1 = X1[i0], v1 = X1[i1];
+                X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
+                X0[i1] = make_float2(u0.x - v0.x, u0.y - v0.y);
+                X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
+                X1[i1] = make_float2(u1.x - v1.x, u1.y - v1.y);
+            }
+            __syncthreads();
+            break;
+        }
     }
     const float inv_n = 1.0f / (float)n;
     float4* dst = p.filt + ((size_t)zb * p.na + a) * p.sq;

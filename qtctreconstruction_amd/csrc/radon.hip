@@ -14,8 +14,8 @@
 // The candidate pixels come from a conservative interval in s (+-1/32 of a
 // bin); the exact floor(s) test decides.
 //
-// Four slices share one lane: the geometry is slice-independent, so the index
-// work is done once and the four values arrive as one float4 load. Rays of
+// Eight slices share one lane: the geometry is slice-independent, so the index
+// work is done once and the eight values arrive as two float4 loads. Rays of
 // angles nearer 0/180 deg walk down columns of the row-major copy; rays of
 // angles nearer 90 deg walk along rows, and read the transposed copy, so that
 // adjacent lanes (adjacent bins) read adjacent addresses in both cases.
@@ -77,22 +77,23 @@ __global__ void minmax_decode_kernel(const unsigned* keys, int nz, float* mm) {
     if (i < 2 * nz) mm[i] = key2f(keys[i]);
 }
 
-// block (32,8) handles a 32x32 pixel tile of one z-block: reads 4 slices,
-// normalises (SerialBackend.cpp:78-80,104,106), writes the row layout
-// directly and the column layout through an LDS transpose.
+// block (32,8) handles a 32x32 pixel tile of one Radon z-block (kRZ = 8
+// slices): reads the slices, normalises (SerialBackend.cpp:78-80,104,106),
+// writes the row layout [zb][y][x][8] directly and the column layout
+// [zb][x][y][8] through an LDS transpose. A pixel is 32 bytes (2 float4).
 __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict__ vol, int w,
                                                         int h, int nz,
                                                         const float* __restrict__ mm,
                                                         float4* __restrict__ rowL,
                                                         float4* __restrict__ colL) {
-    __shared__ float4 tile[32][33];
+    __shared__ float4 tile[32][33][2];
     const int zb = blockIdx.z;
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-    float mn[kZB], inv[kZB];
-    bool live[kZB];
+    float mn[kRZ], inv[kRZ];
+    bool live[kRZ];
 #pragma unroll
-    for (int k = 0; k < kZB; ++k) {
-        const int z = zb * kZB + k;
+    for (int k = 0; k < kRZ; ++k) {
+        const int z = zb * kRZ + k;
         live[k] = z < nz;
         const float lo = live[k] ? mm[2 * z] : 0.0f;
         const float hi = live[k] ? mm[2 * z + 1] : 1.0f;
@@ -104,27 +105,36 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
     const size_t ss = (size_t)w * h;
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int x = bx + threadIdx.x, y = by + r;
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (x < w && y < h) {
-            float t[kZB];
+        float t[kRZ];
 #pragma unroll
-            for (int k = 0; k < kZB; ++k) {
-                float o = 0.0f;
+        for (int k = 0; k < kRZ; ++k) t[k] = 0.0f;
+        if (x < w && y < h) {
+#pragma unroll
+            for (int k = 0; k < kRZ; ++k) {
                 if (live[k]) {
-                    const float raw = vol[(size_t)(zb * kZB + k) * ss + (size_t)y * w + x];
-                    o = (raw == 0.0f) ? 0.0f : (raw - mn[k]) * inv[k];
+                    const float raw = vol[(size_t)(zb * kRZ + k) * ss + (size_t)y * w + x];
+                    t[k] = (raw == 0.0f) ? 0.0f : (raw - mn[k]) * inv[k];
                 }
-                t[k] = o;
             }
-            v = make_float4(t[0], t[1], t[2], t[3]);
-            rowL[((size_t)zb * h + y) * w + x] = v;
         }
-        tile[r][threadIdx.x] = v;
+        const float4 v0 = make_float4(t[0], t[1], t[2], t[3]);
+        const float4 v1 = make_float4(t[4], t[5], t[6], t[7]);
+        if (x < w && y < h) {
+            float4* d = rowL + (((size_t)zb * h + y) * w + x) * 2;
+            d[0] = v0;
+            d[1] = v1;
+        }
+        tile[r][threadIdx.x][0] = v0;
+        tile[r][threadIdx.x][1] = v1;
     }
     __syncthreads();
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int y = by + threadIdx.x, x = bx + r;
-        if (x < w && y < h) colL[((size_t)zb * w + x) * h + y] = tile[threadIdx.x][r];
+        if (x < w && y < h) {
+            float4* d = colL + (((size_t)zb * w + x) * h + y) * 2;
+            d[0] = tile[threadIdx.x][r][0];
+            d[1] = tile[threadIdx.x][r][1];
+        }
     }
 }
 
@@ -146,14 +156,17 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
     const float cx = (float)(w - 1) * 0.5f;
     const float cy = (float)(h - 1) * 0.5f;
     const float dc = (float)(bins / 2);
-    const int db = bins;
     const bool col = fabsf(sn) > fabsf(c);
-    const float4* __restrict__ base =
-        col ? colL + (size_t)zb * w * h : rowL + (size_t)zb * w * h;
+    const float4* __restrict__ base = (col ? colL : rowL) + (size_t)zb * w * h * 2;
+    const size_t sx_stride = col ? (size_t)h * 2 : 2;  // float4 step for x+1
+    const size_t sy_stride = col ? 2 : (size_t)w * 2;  // float4 step for y+1
 
     const float slo = (float)(i - 1) - kDelta - dc;  // s - dc lower bound
     const float shi = (float)(i + 1) + kDelta - dc;  // s - dc upper bound
     const float ac = fabsf(c), as = fabsf(sn);
+    // factor rules of SerialBackend.cpp:111-120 for this bin
+    const bool last = (i == bins - 1);
+    const int im1 = (i >= 1) ? i - 1 : -0x7fffffff;
 
     // rows that can hold a candidate: (y-cy)*sn in [slo - cx|c|, shi + cx|c|]
     int y0 = 0, y1 = h - 1;
@@ -167,43 +180,63 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
         y1 = min(h - 1, (int)ceilf(fminf(yb, (float)h + 1.0f)) + 1);
     }
 
-    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    // Per-row candidate interval in x: x - cx = ((s-dc) - (y-cy) sn) / c over
+    // s - dc in [slo, shi]. For |c| >= |s| it is stepped from row to row,
+    // x in [xa0 + y*dxa, xa0 + y*dxa + span] (|dxa| <= 1, |1/c| <= sqrt2, so
+    // the stepping error, ~1e-3 px, is far below the kDelta/|c| margin);
+    // steeper rays compute it per row from sy, whose error then scales with
+    // the distance to the strip and vanishes where candidates exist.
+    const bool has_c = ac > 1e-20f;
+    const float inv_c = has_c ? 1.0f / c : 0.0f;
+    float xa0 = 0.0f, dxa = 0.0f, span = (float)w;
+    if (!col) {
+        const float e0 = (slo + cy * sn) * inv_c, e1 = (shi + cy * sn) * inv_c;
+        xa0 = cx + fminf(e0, e1);
+        span = fabsf(e1 - e0);
+        dxa = -sn * inv_c;
+    }
+
+    float acc[kRZ];
+#pragma unroll
+    for (int k = 0; k < kRZ; ++k) acc[k] = 0.0f;
     for (int y = y0; y <= y1; ++y) {
         const float sy = ((float)y - cy) * sn;
         int x0 = 0, x1 = w - 1;
-        if (ac > 1e-20f) {
-            const float lo = (slo - sy) / c;
-            const float hi = (shi - sy) / c;
-            const float xa = cx + fminf(lo, hi), xb = cx + fmaxf(lo, hi);
-            x0 = max(0, (int)ceilf(fmaxf(xa, -1.0f)));
-            x1 = min(w - 1, (int)floorf(fminf(xb, (float)w)));
+        if (!col) {
+            const float xa = xa0 + (float)y * dxa;
+            x0 = max(0, (int)ceilf(fmaxf(xa - 0.01f, -1.0f)));
+            x1 = min(w - 1, (int)floorf(fminf(xa + span + 0.01f, (float)w)));
+        } else if (has_c) {
+            const float lo = (slo - sy) * inv_c;
+            const float hi = (shi - sy) * inv_c;
+            x0 = max(0, (int)ceilf(fmaxf(cx + fminf(lo, hi), -1.0f)));
+            x1 = min(w - 1, (int)floorf(fminf(cx + fmaxf(lo, hi), (float)w)));
         }
-        for (int x = x0; x <= x1; ++x) {
+        const float4* p = base + (size_t)y * sy_stride + (size_t)x0 * sx_stride;
+        float xmc = (float)x0 - cx;
+        for (int x = x0; x <= x1; ++x, xmc += 1.0f, p += sx_stride) {
             // SerialBackend.cpp:110-120, same float operations and order
-            const float sx = ((float)x - cx) * c;
-            const float s = sx + sy + dc;
+            const float s = xmc * c + sy + dc;
             const float fl = floorf(s);
             const int i0 = (int)fl;
             const float frac = s - fl;
-            float fac;
-            if (i0 == i) {
-                fac = (i < db - 1) ? (1.0f - frac) : 1.0f;
-            } else if (i0 == i - 1 && i >= 1) {
-                fac = frac;
-            } else {
-                continue;
-            }
-            const float4 v = col ? base[(size_t)x * h + y] : base[(size_t)y * w + x];
-            a0 += v.x * fac;
-            a1 += v.y * fac;
-            a2 += v.z * fac;
-            a3 += v.w * fac;
+            const float f_here = last ? 1.0f : 1.0f - frac;
+            // non-candidates add +0, which leaves every partial sum unchanged
+            const float fac = (i0 == i) ? f_here : ((i0 == im1) ? frac : 0.0f);
+            const float4 v0 = p[0], v1 = p[1];
+            acc[0] += v0.x * fac;
+            acc[1] += v0.y * fac;
+            acc[2] += v0.z * fac;
+            acc[3] += v0.w * fac;
+            acc[4] += v1.x * fac;
+            acc[5] += v1.y * fac;
+            acc[6] += v1.z * fac;
+            acc[7] += v1.w * fac;
         }
     }
-    const float acc[4] = {a0, a1, a2, a3};
 #pragma unroll
-    for (int k = 0; k < kZB; ++k) {
-        const int z = zb * kZB + k;
+    for (int k = 0; k < kRZ; ++k) {
+        const int z = zb * kRZ + k;
         if (z < nz) sino[((size_t)z * na + a) * bins + i] = acc[k];
     }
 }
@@ -230,7 +263,7 @@ hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hip
 
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, hipStream_t s) {
-    const int nzb = (nz + kZB - 1) / kZB;
+    const int nzb = (nz + kRZ - 1) / kRZ;
     dim3 grid((w + 31) / 32, (h + 31) / 32, nzb);
     hipLaunchKernelGGL(normalise_kernel, grid, dim3(32, 8), 0, s, d_vol, w, h, nz, d_mm, d_row,
                        d_col);
@@ -240,7 +273,7 @@ hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const floa
 hipError_t launch_radon(const float4* d_row, const float4* d_col, int w, int h, int nz, int na,
                         int bins, const float* d_cos, const float* d_sin, float* d_sino,
                         hipStream_t s) {
-    const int nzb = (nz + kZB - 1) / kZB;
+    const int nzb = (nz + kRZ - 1) / kRZ;
     dim3 grid((bins + 255) / 256, na, nzb);
     hipLaunchKernelGGL(radon_ray_kernel, grid, dim3(256), 0, s, d_row, d_col, w, h, nz, na, bins,
                        d_cos, d_sin, d_sino);
