@@ -41,13 +41,13 @@ void slot_init(ct_hip_ctx* c, BatchSlot* s, bool own_stream) {
     hip_check(hipEventCreate(&s->ev_done), "slot event");
     s->ftab = static_cast<FiltJob*>(dmalloc(c, sizeof(FiltJob) * kMaxBatch));
     s->btab = static_cast<BpJob*>(dmalloc(c, sizeof(BpJob) * kMaxBatch));
-    s->hdr = static_cast<int*>(dmalloc(c, sizeof(int) * (2 + 2 * kMaxFftClasses)));
+    s->hdr = static_cast<int*>(dmalloc(c, sizeof(int) * (2 + 3 * kMaxFftClasses)));
     s->counters = static_cast<int*>(dmalloc(c, sizeof(int) * (1 + kMaxFftClasses)));
     hip_check(hipHostMalloc((void**)&s->h_ftab, sizeof(FiltJob) * kMaxBatch, hipHostMallocDefault),
               "pinned table");
     hip_check(hipHostMalloc((void**)&s->h_btab, sizeof(BpJob) * kMaxBatch, hipHostMallocDefault),
               "pinned table");
-    hip_check(hipHostMalloc((void**)&s->h_hdr, sizeof(int) * (2 + 2 * kMaxFftClasses),
+    hip_check(hipHostMalloc((void**)&s->h_hdr, sizeof(int) * (2 + 3 * kMaxFftClasses),
                             hipHostMallocDefault),
               "pinned hdr");
     s->ready = true;
@@ -270,8 +270,9 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
             items += ((jobs[i].nz + kZB - 1) / kZB) * jobs[i].na;
             ++pos;
         }
-        s->h_hdr[2 + 2 * ci] = (int)kv.second.size();
-        s->h_hdr[3 + 2 * ci] = items;
+        s->h_hdr[2 + 3 * ci] = cls_first.back();
+        s->h_hdr[3 + 3 * ci] = (int)kv.second.size();
+        s->h_hdr[4 + 3 * ci] = items;
         ++ci;
     }
     s->h_hdr[0] = nj;
@@ -282,7 +283,7 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
     hip_check(hipMemcpyAsync(s->btab, s->h_btab, sizeof(BpJob) * nj, hipMemcpyHostToDevice,
                              s->stream),
               "upload backprojection work list");
-    hip_check(hipMemcpyAsync(s->hdr, s->h_hdr, sizeof(int) * (2 + 2 * ci), hipMemcpyHostToDevice,
+    hip_check(hipMemcpyAsync(s->hdr, s->h_hdr, sizeof(int) * (2 + 3 * ci), hipMemcpyHostToDevice,
                              s->stream),
               "upload header");
     std::vector<int> fgrid(ci);
@@ -294,8 +295,10 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
     need(bgrid > 0, "fbpBatch: no occupancy for the persistent backprojection");
     auto enqueue = [&]() {
         hip_check(launch_zero_ints(s->counters, 1 + ci, s->stream), "zero counters");
+        // kernel arguments depend only on the graph key (class count, FFT
+        // lengths, grids); where each class's jobs start is in the header
         for (int k = 0; k < ci; ++k)
-            hip_check(launch_row_filter_persistent(s->ftab + cls_first[k], s->hdr + 2 + 2 * k,
+            hip_check(launch_row_filter_persistent(s->ftab, s->hdr + 2 + 3 * k,
                                                    s->counters + 1 + k, cls_nfft[k], fgrid[k],
                                                    s->stream),
                       "persistent row filter");

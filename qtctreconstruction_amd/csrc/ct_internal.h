@@ -70,7 +70,8 @@ FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, i
                         const float* d_H, const float2* d_tw, float4* d_filt);
 size_t fft_lds_bytes(int nfft);
 int filter_persistent_grid(int max_nfft, int device);
-// hdr = {njobs, total items}; counter must be 0 at launch
+// hdr = {first job index in d_jobs, njobs, total items}; counter must be 0
+// at launch. Everything batch-dependent is read from device memory.
 hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
                                         int max_nfft, int grid, hipStream_t s);
 hipError_t launch_filter_design(int n, int type, float* d_H, hipStream_t s);

@@ -208,7 +208,10 @@ __global__ __launch_bounds__(256) void row_filter_persistent_kernel(const FiltJo
                                                                     int* __restrict__ counter) {
     extern __shared__ float2 sm[];
     __shared__ int item_s;
-    const int njobs = hdr[0], total = hdr[1];
+    // hdr = {first job of this launch's list, job count, item count}: read
+    // from device memory so a captured launch stays valid for any batch
+    jobs += hdr[0];
+    const int njobs = hdr[1], total = hdr[2];
     for (;;) {
         if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
         __syncthreads();

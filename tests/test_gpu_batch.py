@@ -86,3 +86,29 @@ def test_autotune_returns_a_timed_config():
     best, ms = ctx.autotune(256, 180)
     assert 0 <= best < len(ctx.bp_configs())
     assert all(t > 0 for t in ms) and ms[best] == min(ms)
+
+
+def test_graph_replay_with_changing_class_composition(O):
+    """Batches with the same FFT-length classes but different numbers of jobs
+    per class replay one captured graph; each volume must still equal its
+    standalone reconstruction (the per-class work-list offsets live in device
+    memory, not in the captured kernel arguments)."""
+    be = ct.HipBackend(0)
+    ctx = be.ctx
+    small = [(40, 3, 24), (48, 2, 30), (44, 4, 18)]   # nfft 128
+    large = [(96, 2, 36), (100, 3, 20)]               # nfft 512 / 512
+    pool = {}
+    for k, (N, nz, na) in enumerate(small + large):
+        s = _make_sinos(O, 300 + k, N, nz, na)
+        pool[(N, nz, na)] = (s, ctx.alloc(s.nbytes).upload(s), ctx.alloc(4 * nz * N * N))
+    compositions = [small[:1] + large, small + large[:1], small[:2] + large, small + large]
+    for comp in compositions:
+        jobs = []
+        for (N, nz, na) in comp:
+            s, ds, do = pool[(N, nz, na)]
+            jobs.append((ds, do, O.serial_angles(na), nz, na, N, ct.FilterType.Ramp))
+        ctx.fbp_batch(jobs, ct.CT_BATCH_GRAPH)
+        for (N, nz, na) in comp:
+            s, ds, do = pool[(N, nz, na)]
+            ref = _alone(be, s, O.serial_angles(na), ct.FilterType.Ramp)
+            assert np.array_equal(do.download((nz, N, N)), ref), (comp, N)

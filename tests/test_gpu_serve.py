@@ -113,8 +113,8 @@ def test_admission_by_budget_and_overlap(inputs):
 
 def test_oversize_job_fails_cleanly(inputs):
     ctx, jobs = inputs
-    j = jobs[3]
-    srv = ct.Server(pool_bytes=1 << 20, max_batch=4, slots=1)
+    j = jobs[3]  # needs ~196 KB of output + ~349 KB of filtered sinogram
+    srv = ct.Server(pool_bytes=256 << 10, max_batch=4, slots=1)
     t = srv.submit(j["dev"], j["ang"], j["nz"], j["na"], j["N"], j["f"])
     with pytest.raises(ct.BackendError):
         srv.wait(30000)
