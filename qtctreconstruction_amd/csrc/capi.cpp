@@ -113,7 +113,7 @@ float* get_filter(ct_hip_ctx* c, int nfft, int type) {
 float2* get_twiddles(ct_hip_ctx* c, int nfft) {
     auto it = c->twiddles.find(nfft);
     if (it != c->twiddles.end()) return it->second;
-    float2* t = static_cast<float2*>(dmalloc(c, sizeof(float2) * (nfft / 2)));
+    float2* t = static_cast<float2*>(dmalloc(c, sizeof(float2) * (nfft - 1)));
     hip_check(launch_twiddles(nfft, t, c->stream), "twiddles");
     c->twiddles[nfft] = t;
     return t;
@@ -145,8 +145,10 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
                  float* d_sino, float* d_mm) {
     unsigned* keys = ensure<unsigned>(c, c->keys, sizeof(unsigned) * 2 * (size_t)nz);
     const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
-    float4* row = ensure<float4>(c, c->row, sizeof(float4) * 2 * rzb * (size_t)w * h);
-    float4* col = ensure<float4>(c, c->col, sizeof(float4) * 2 * rzb * (size_t)w * h);
+    // per z-block h x (w + kPadX) pixels of 2 float4 (zero pad included)
+    const size_t lay = sizeof(float4) * 2 * rzb * (size_t)(w + kPadX) * h;
+    float4* row = ensure<float4>(c, c->row, lay);
+    float4* col = ensure<float4>(c, c->col, lay);
     hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");

@@ -17,6 +17,9 @@ namespace cthip {
 constexpr int kZB = 4;
 // Slices per Radon z-block (the normalised copies are [zb][..][..][8]).
 constexpr int kRZ = 8;
+// zero pixels after each row (row layout) and columns after the last column
+// (column layout) of the Radon stage's normalised copies
+constexpr int kPadX = 4;
 
 // Largest FFT length of the LDS row filter (bins <= 1448).
 constexpr int kMaxFFT = 4096;
@@ -58,7 +61,7 @@ struct FiltArgs {
     const float* sino;  // [nz][na][bins]
     int nz, na, bins, sq, pad, nfft, logn;
     const float* H;     // nfft/2+1 taps
-    const float2* tw;   // nfft/2 twiddles
+    const float2* tw;   // nfft-1 twiddles, stage order (W_m^j at m/2-1+j)
     float4* filt;       // [nzb][na][sq] float4
 };
 struct FiltJob {

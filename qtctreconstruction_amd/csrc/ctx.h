@@ -70,7 +70,7 @@ struct ct_hip_ctx {
     Buf vol, row, col, sino, filt, out, keys, mm, ell, pc_cnt, pc_pts, pc_coord;
     // cached device state
     std::map<std::pair<int, int>, float*> filters;  // (nfft, type) -> H[nfft/2+1]
-    std::map<int, float2*> twiddles;                // nfft -> tw[nfft/2]
+    std::map<int, float2*> twiddles;                // nfft -> tw[nfft-1], stage order
     struct Trig {
         float* cos = nullptr;
         float* sin = nullptr;

@@ -18,7 +18,9 @@
 // back separated. A block owns one angle of one z-block (4 rows -> 2 complex
 // FFTs) held in LDS: decimation-in-time forward from bit-reversed input,
 // multiply by H in natural order, decimation-in-frequency inverse whose
-// output is read back at bit-reversed positions. Twiddles live in LDS.
+// output is read back at bit-reversed positions. Twiddles are read from a
+// stage-ordered table in global memory (small enough to stay in L1), at
+// consecutive addresses across a wave.
 // Output is written z-innermost (float4 = 4 slices per bin), the layout the
 // backprojection stages from.
 #include "ct_internal.h"
@@ -59,11 +61,19 @@ __global__ void filter_design_kernel(int n, int type, float* __restrict__ H) {
     H[k] = (float)h;
 }
 
+// Twiddles in stage order: the table of a butterfly span m (m = 2, 4, ..., n)
+// starts at m/2 - 1 and holds W_m^j = exp(-2 pi i j / m) for j < m/2, so a
+// stage reads its twiddles at consecutive addresses. W_m^j equals
+// W_n^(j n/m) bit for bit: the FP64 angles differ by exact power-of-two
+// scalings only.
 __global__ void twiddle_kernel(int n, float2* __restrict__ tw) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n / 2) return;
-    const double a = -2.0 * 3.14159265358979323846 * (double)k / (double)n;
-    tw[k] = make_float2((float)cos(a), (float)sin(a));
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n - 1) return;
+    const int lg = 31 - __clz(idx + 1);  // floor(log2(idx + 1))
+    const int m = 2 << lg;
+    const int j = idx + 1 - m / 2;
+    const double a = -2.0 * 3.14159265358979323846 * (double)j / (double)m;
+    tw[idx] = make_float2((float)cos(a), (float)sin(a));
 }
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -79,10 +89,9 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     const int n = p.nfft, logn = p.logn;
     float2* X0 = sm;
     float2* X1 = sm + n;
-    float2* T = sm + 2 * n;
+    const float2* __restrict__ T = p.tw;  // stage-ordered, L1-resident
     const int tid = threadIdx.x;
     const int nh = n / 2;
-    for (int k = tid; k < nh; k += 256) T[k] = p.tw[k];
     const float* rows[kZB];
 #pragma unroll
     for (int r = 0; r < kZB; ++r) {
@@ -120,11 +129,10 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     }
     const int nq = n / 4;
     for (; h < n; h <<= 2) {
-        const int s1 = n / (2 * h), s2 = n / (4 * h);
         for (int j = tid; j < nq; j += 256) {
             const int g = j / h, k = j - g * h;
             const int e0 = g * 4 * h + k, e1 = e0 + h, e2 = e0 + 2 * h, e3 = e0 + 3 * h;
-            const float2 w1 = T[k * s1], w2 = T[k * s2], w3 = T[(k + h) * s2];
+            const float2 w1 = T[h - 1 + k], w2 = T[2 * h - 1 + k], w3 = T[3 * h - 1 + k];
 #pragma unroll
             for (int f = 0; f < 2; ++f) {
                 float2* X = f ? X1 : X0;
@@ -152,11 +160,10 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     // output; stages of half-span 2h and h fused in pairs, then (odd count)
     // one final radix-2 stage of half-span 1.
     for (h = n / 4; h >= 1; h >>= 2) {
-        const int s1 = n / (2 * h), s2 = n / (4 * h);
         for (int j = tid; j < nq; j += 256) {
             const int g = j / h, k = j - g * h;
             const int e0 = g * 4 * h + k, e1 = e0 + h, e2 = e0 + 2 * h, e3 = e0 + 3 * h;
-            const float2 w2 = T[k * s2], w3 = T[(k + h) * s2], w1 = T[k * s1];
+            const float2 w2 = T[2 * h - 1 + k], w3 = T[3 * h - 1 + k], w1 = T[h - 1 + k];
 #pragma unroll
             for (int f = 0; f < 2; ++f) {
                 float2* X = f ? X1 : X0;
@@ -227,7 +234,7 @@ __global__ __launch_bounds__(256) void row_filter_persistent_kernel(const FiltJo
     }
 }
 
-size_t fft_lds_bytes(int nfft) { return sizeof(float2) * (size_t)(2 * nfft + nfft / 2); }
+size_t fft_lds_bytes(int nfft) { return sizeof(float2) * (size_t)(2 * nfft); }
 
 static hipError_t set_lds_attr(const void* fn) {
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -266,7 +273,7 @@ hipError_t launch_filter_design(int n, int type, float* d_H, hipStream_t s) {
 }
 
 hipError_t launch_twiddles(int n, float2* d_tw, hipStream_t s) {
-    const int cnt = n / 2;
+    const int cnt = n - 1;
     hipLaunchKernelGGL(twiddle_kernel, dim3((cnt + 127) / 128), dim3(128), 0, s, n, d_tw);
     return hipGetLastError();
 }

@@ -21,6 +21,9 @@
 // adjacent lanes (adjacent bins) read adjacent addresses in both cases.
 #include "ct_internal.h"
 
+#include <algorithm>
+#include <cstdlib>
+
 namespace cthip {
 
 __device__ __forceinline__ unsigned f2key(float f) {
@@ -79,8 +82,13 @@ __global__ void minmax_decode_kernel(const unsigned* keys, int nz, float* mm) {
 
 // block (32,8) handles a 32x32 pixel tile of one Radon z-block (kRZ = 8
 // slices): reads the slices, normalises (SerialBackend.cpp:78-80,104,106),
-// writes the row layout [zb][y][x][8] directly and the column layout
-// [zb][x][y][8] through an LDS transpose. A pixel is 32 bytes (2 float4).
+// writes the row layout directly and the column layout through an LDS
+// transpose. A pixel is 32 bytes (the 8 slices as 2 float4):
+//   row layout    rowL[(((zb*h) + y)*(w + kPadX) + x)*2 + {0,1}]
+//   column layout colL[(((zb*(w + kPadX)) + x)*h + y)*2 + {0,1}]
+// x runs to w + kPadX - 1: the kPadX pixels past each row (columns past the
+// last column) are written as zeros, so a ray's fixed-size chunk of
+// candidates may run past the edge of the image and read only zeros.
 __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict__ vol, int w,
                                                         int h, int nz,
                                                         const float* __restrict__ mm,
@@ -89,6 +97,7 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
     __shared__ float4 tile[32][33][2];
     const int zb = blockIdx.z;
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int wp = w + kPadX;
     float mn[kRZ], inv[kRZ];
     bool live[kRZ];
 #pragma unroll
@@ -103,6 +112,8 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
         inv[k] = 1.0f / span;
     }
     const size_t ss = (size_t)w * h;
+    float4* __restrict__ r0 = rowL + (size_t)zb * h * wp * 2;
+    float4* __restrict__ c0 = colL + (size_t)zb * h * wp * 2;
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int x = bx + threadIdx.x, y = by + r;
         float t[kRZ];
@@ -119,8 +130,8 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
         }
         const float4 v0 = make_float4(t[0], t[1], t[2], t[3]);
         const float4 v1 = make_float4(t[4], t[5], t[6], t[7]);
-        if (x < w && y < h) {
-            float4* d = rowL + (((size_t)zb * h + y) * w + x) * 2;
+        if (x < wp && y < h) {  // pixels w..w+kPadX-1 are the zero pad
+            float4* d = r0 + ((size_t)y * wp + x) * 2;
             d[0] = v0;
             d[1] = v1;
         }
@@ -130,8 +141,8 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
     __syncthreads();
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int y = by + threadIdx.x, x = bx + r;
-        if (x < w && y < h) {
-            float4* d = colL + (((size_t)zb * w + x) * h + y) * 2;
+        if (x < wp && y < h) {
+            float4* d = c0 + ((size_t)x * h + y) * 2;
             d[0] = tile[threadIdx.x][r][0];
             d[1] = tile[threadIdx.x][r][1];
         }
@@ -144,38 +155,134 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
 // floor(s_float) in {i-1, i}.
 constexpr float kDelta = 1.0f / 32.0f;
 
+// Ray walk. A lane's candidates come in chunks of U consecutive pixels of
+// one row, in Serial's raster order: (row y, x..x+U-1).
+//
+// A chunk may reach past the last candidate x1 of its row: such pixels are
+// either inside the image and outside the conservative s-interval (their
+// floor(s) is neither i-1 nor i, so they add v*0), or they are zero pad
+// pixels (x >= w). Either way every partial sum is left unchanged, so no
+// per-candidate mask is needed; chunk starts are clamped to x <= w, which
+// keeps every position within the kPadX pad.
+struct RayGeom {
+    float c, sn, cx, cy, dc, slo, shi, inv_c;
+    float xa0, dxa;  // row-stepped interval start (|c| >= |s|)
+    bool has_c;
+    int w, i, im1;
+    float keep;      // 0 for the last bin (full weight), else 1
+};
+
+template <bool STEEP>
+__device__ __forceinline__ void row_setup(const RayGeom& g, int y, int& x0, int& x1, float& sy) {
+    sy = ((float)y - g.cy) * g.sn;
+    if (!STEEP) {
+        // at most 3 candidates per row (span + 0.02 <= 2.0625*sqrt2 + 0.02 < 3)
+        const float xa = g.xa0 + (float)y * g.dxa;
+        x0 = min(max(0, (int)ceilf(fmaxf(xa - 0.01f, -1.0f))), g.w);
+        x1 = x0;
+    } else {
+        x0 = 0;
+        x1 = g.w - 1;
+        if (g.has_c) {
+            const float lo = (g.slo - sy) * g.inv_c;
+            const float hi = (g.shi - sy) * g.inv_c;
+            x0 = max(0, (int)ceilf(fmaxf(g.cx + fminf(lo, hi), -1.0f)));
+            x1 = min(g.w - 1, (int)floorf(fminf(g.cx + fmaxf(lo, hi), (float)g.w)));
+            x0 = min(x0, g.w);
+        }
+    }
+}
+
+template <int U>
+__device__ __forceinline__ void chunk_load(const char* __restrict__ p, unsigned off,
+                                           unsigned xstep, float4 (&v)[U][2]) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        v[k][0] = *reinterpret_cast<const float4*>(p + off + k * xstep);
+        v[k][1] = *reinterpret_cast<const float4*>(p + off + k * xstep + 16);
+    }
+}
+
+template <int U>
+__device__ __forceinline__ void chunk_add(const RayGeom& g, int x, float sy, const float4 (&v)[U][2],
+                                          float (&acc)[kRZ]) {
+    const float xm0 = (float)x - g.cx;
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+        // SerialBackend.cpp:110-120, same float operations and order: s, its
+        // floor and fraction (exact for s >= 0; s < 0 gives i0 <= -1, which
+        // no lane's candidate test accepts), and the two bin factors
+        const float s = (xm0 + (float)k) * g.c + sy + g.dc;
+        const int i0 = (int)floorf(s);
+        const float frac = __builtin_amdgcn_fractf(s);
+        const float f_here = fmaf(-frac, g.keep, 1.0f);  // 1 - frac, or 1 for the last bin
+        // non-candidates add +0, which leaves every partial sum unchanged
+        const float fac = (i0 == g.i) ? f_here : ((i0 == g.im1) ? frac : 0.0f);
+        acc[0] += v[k][0].x * fac;
+        acc[1] += v[k][0].y * fac;
+        acc[2] += v[k][0].z * fac;
+        acc[3] += v[k][0].w * fac;
+        acc[4] += v[k][1].x * fac;
+        acc[5] += v[k][1].y * fac;
+        acc[6] += v[k][1].z * fac;
+        acc[7] += v[k][1].w * fac;
+    }
+}
+
+// A lane's walk: rows y0..y1 in order, each as chunks of U pixels from its
+// first candidate. All loads of a chunk are issued before its arithmetic.
+template <bool STEEP, int U>
+__device__ __forceinline__ void ray_walk(const RayGeom& g, const char* __restrict__ p,
+                                         unsigned ystep,
+                                         unsigned xstep, int y0, int y1, float (&acc)[kRZ]) {
+    for (int y = y0; y <= y1; ++y) {
+        int x0, x1;
+        float sy;
+        row_setup<STEEP>(g, y, x0, x1, sy);
+        const unsigned roff = (unsigned)y * ystep;
+        for (int x = x0; x <= x1; x += U) {
+            float4 v[U][2];
+            chunk_load<U>(p, roff + (unsigned)x * xstep, xstep, v);
+            chunk_add<U>(g, x, sy, v, acc);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void radon_ray_kernel(
     const float4* __restrict__ rowL, const float4* __restrict__ colL, int w, int h, int nz,
     int na, int bins, const float* __restrict__ cos_t, const float* __restrict__ sin_t,
-    float* __restrict__ sino) {
+    float* __restrict__ sino, int zb0) {
     const int i = blockIdx.x * 256 + threadIdx.x;  // bin owned by this lane
     const int a = blockIdx.y;
-    const int zb = blockIdx.z;
+    const int zb = zb0 + blockIdx.z;
     if (i >= bins) return;
-    const float c = cos_t[a], sn = sin_t[a];
-    const float cx = (float)(w - 1) * 0.5f;
-    const float cy = (float)(h - 1) * 0.5f;
-    const float dc = (float)(bins / 2);
-    const bool col = fabsf(sn) > fabsf(c);
-    const float4* __restrict__ base = (col ? colL : rowL) + (size_t)zb * w * h * 2;
-    const size_t sx_stride = col ? (size_t)h * 2 : 2;  // float4 step for x+1
-    const size_t sy_stride = col ? 2 : (size_t)w * 2;  // float4 step for y+1
+    RayGeom g;
+    g.c = cos_t[a];
+    g.sn = sin_t[a];
+    g.cx = (float)(w - 1) * 0.5f;
+    g.cy = (float)(h - 1) * 0.5f;
+    g.dc = (float)(bins / 2);
+    g.w = w;
+    g.i = i;
+    g.im1 = (i >= 1) ? i - 1 : -0x7fffffff;
+    g.keep = (i == bins - 1) ? 0.0f : 1.0f;  // SerialBackend.cpp:113-116
+    const bool col = fabsf(g.sn) > fabsf(g.c);
+    // one z-block's copy; byte offsets within it fit 32 bits
+    const size_t wp = (size_t)(w + kPadX);
+    const char* p = reinterpret_cast<const char*>((col ? colL : rowL) + (size_t)zb * h * wp * 2);
 
-    const float slo = (float)(i - 1) - kDelta - dc;  // s - dc lower bound
-    const float shi = (float)(i + 1) + kDelta - dc;  // s - dc upper bound
-    const float ac = fabsf(c), as = fabsf(sn);
-    // factor rules of SerialBackend.cpp:111-120 for this bin
-    const bool last = (i == bins - 1);
-    const int im1 = (i >= 1) ? i - 1 : -0x7fffffff;
+    g.slo = (float)(i - 1) - kDelta - g.dc;  // s - dc lower bound
+    g.shi = (float)(i + 1) + kDelta - g.dc;  // s - dc upper bound
+    const float ac = fabsf(g.c), as = fabsf(g.sn);
 
     // rows that can hold a candidate: (y-cy)*sn in [slo - cx|c|, shi + cx|c|]
     int y0 = 0, y1 = h - 1;
     if (as > 1e-20f) {
-        const float lo = (slo - cx * ac) / as;
-        const float hi = (shi + cx * ac) / as;
+        const float lo = (g.slo - g.cx * ac) / as;
+        const float hi = (g.shi + g.cx * ac) / as;
         float ya, yb;
-        if (sn > 0.0f) { ya = cy + lo; yb = cy + hi; }
-        else           { ya = cy - hi; yb = cy - lo; }
+        if (g.sn > 0.0f) { ya = g.cy + lo; yb = g.cy + hi; }
+        else             { ya = g.cy - hi; yb = g.cy - lo; }
         y0 = max(0, (int)floorf(fmaxf(ya, -2.0f)) - 1);
         y1 = min(h - 1, (int)ceilf(fminf(yb, (float)h + 1.0f)) + 1);
     }
@@ -183,57 +290,26 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
     // Per-row candidate interval in x: x - cx = ((s-dc) - (y-cy) sn) / c over
     // s - dc in [slo, shi]. For |c| >= |s| it is stepped from row to row,
     // x in [xa0 + y*dxa, xa0 + y*dxa + span] (|dxa| <= 1, |1/c| <= sqrt2, so
-    // the stepping error, ~1e-3 px, is far below the kDelta/|c| margin);
-    // steeper rays compute it per row from sy, whose error then scales with
+    // the stepping error, ~1e-3 px, is far below the kDelta/|c| margin).
+    // Steeper rays compute it per row from sy, whose error then scales with
     // the distance to the strip and vanishes where candidates exist.
-    const bool has_c = ac > 1e-20f;
-    const float inv_c = has_c ? 1.0f / c : 0.0f;
-    float xa0 = 0.0f, dxa = 0.0f, span = (float)w;
+    g.has_c = ac > 1e-20f;
+    g.inv_c = g.has_c ? 1.0f / g.c : 0.0f;
+    g.xa0 = 0.0f;
+    g.dxa = 0.0f;
     if (!col) {
-        const float e0 = (slo + cy * sn) * inv_c, e1 = (shi + cy * sn) * inv_c;
-        xa0 = cx + fminf(e0, e1);
-        span = fabsf(e1 - e0);
-        dxa = -sn * inv_c;
+        const float e0 = (g.slo + g.cy * g.sn) * g.inv_c, e1 = (g.shi + g.cy * g.sn) * g.inv_c;
+        g.xa0 = g.cx + fminf(e0, e1);
+        g.dxa = -g.sn * g.inv_c;
     }
 
     float acc[kRZ];
 #pragma unroll
     for (int k = 0; k < kRZ; ++k) acc[k] = 0.0f;
-    for (int y = y0; y <= y1; ++y) {
-        const float sy = ((float)y - cy) * sn;
-        int x0 = 0, x1 = w - 1;
-        if (!col) {
-            const float xa = xa0 + (float)y * dxa;
-            x0 = max(0, (int)ceilf(fmaxf(xa - 0.01f, -1.0f)));
-            x1 = min(w - 1, (int)floorf(fminf(xa + span + 0.01f, (float)w)));
-        } else if (has_c) {
-            const float lo = (slo - sy) * inv_c;
-            const float hi = (shi - sy) * inv_c;
-            x0 = max(0, (int)ceilf(fmaxf(cx + fminf(lo, hi), -1.0f)));
-            x1 = min(w - 1, (int)floorf(fminf(cx + fmaxf(lo, hi), (float)w)));
-        }
-        const float4* p = base + (size_t)y * sy_stride + (size_t)x0 * sx_stride;
-        float xmc = (float)x0 - cx;
-        for (int x = x0; x <= x1; ++x, xmc += 1.0f, p += sx_stride) {
-            // SerialBackend.cpp:110-120, same float operations and order
-            const float s = xmc * c + sy + dc;
-            const float fl = floorf(s);
-            const int i0 = (int)fl;
-            const float frac = s - fl;
-            const float f_here = last ? 1.0f : 1.0f - frac;
-            // non-candidates add +0, which leaves every partial sum unchanged
-            const float fac = (i0 == i) ? f_here : ((i0 == im1) ? frac : 0.0f);
-            const float4 v0 = p[0], v1 = p[1];
-            acc[0] += v0.x * fac;
-            acc[1] += v0.y * fac;
-            acc[2] += v0.z * fac;
-            acc[3] += v0.w * fac;
-            acc[4] += v1.x * fac;
-            acc[5] += v1.y * fac;
-            acc[6] += v1.z * fac;
-            acc[7] += v1.w * fac;
-        }
-    }
+    if (!col)  // row layout: x steps 32 bytes, y steps 32 (w + kPadX)
+        ray_walk<false, 3>(g, p, 32u * (unsigned)wp, 32u, y0, y1, acc);
+    else       // column layout: x steps 32 h bytes, y steps 32
+        ray_walk<true, 3>(g, p, 32u, 32u * (unsigned)h, y0, y1, acc);
 #pragma unroll
     for (int k = 0; k < kRZ; ++k) {
         const int z = zb * kRZ + k;
@@ -264,7 +340,7 @@ hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hip
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, hipStream_t s) {
     const int nzb = (nz + kRZ - 1) / kRZ;
-    dim3 grid((w + 31) / 32, (h + 31) / 32, nzb);
+    dim3 grid((w + kPadX + 31) / 32, (h + 31) / 32, nzb);
     hipLaunchKernelGGL(normalise_kernel, grid, dim3(32, 8), 0, s, d_vol, w, h, nz, d_mm, d_row,
                        d_col);
     return hipGetLastError();
@@ -274,10 +350,19 @@ hipError_t launch_radon(const float4* d_row, const float4* d_col, int w, int h, 
                         int bins, const float* d_cos, const float* d_sin, float* d_sino,
                         hipStream_t s) {
     const int nzb = (nz + kRZ - 1) / kRZ;
-    dim3 grid((bins + 255) / 256, na, nzb);
-    hipLaunchKernelGGL(radon_ray_kernel, grid, dim3(256), 0, s, d_row, d_col, w, h, nz, na, bins,
-                       d_cos, d_sin, d_sino);
-    return hipGetLastError();
+    int per = nzb;  // z-blocks per launch
+    if (const char* env = std::getenv("CT_HIP_RADON_ZB_PER_LAUNCH")) {
+        const int v = std::atoi(env);
+        if (v >= 1) per = v;
+    }
+    for (int zb0 = 0; zb0 < nzb; zb0 += per) {
+        dim3 grid((bins + 255) / 256, na, std::min(per, nzb - zb0));
+        hipLaunchKernelGGL(radon_ray_kernel, grid, dim3(256), 0, s, d_row, d_col, w, h, nz, na,
+                           bins, d_cos, d_sin, d_sino, zb0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace cthip
