@@ -77,13 +77,27 @@ ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
         cs[a] = std::cos(th);
         sn[a] = std::sin(th);
     }
+    // the Radon stage walks rays of angles with |sin| <= |cos| along rows and
+    // the others along columns (the same float test as the kernels')
+    std::vector<int> alist;
+    alist.reserve(na);
+    for (int a = 0; a < na; ++a)
+        if (!(std::fabs(sn[a]) > std::fabs(cs[a]))) alist.push_back(a);
+    const int n_row = (int)alist.size();
+    for (int a = 0; a < na; ++a)
+        if (std::fabs(sn[a]) > std::fabs(cs[a])) alist.push_back(a);
     ct_hip_ctx::Trig t;
     t.cos = static_cast<float*>(dmalloc(c, sizeof(float) * na));
     t.sin = static_cast<float*>(dmalloc(c, sizeof(float) * na));
+    t.alist = static_cast<int*>(dmalloc(c, sizeof(int) * na));
+    t.n_row = n_row;
     hip_check(hipMemcpyAsync(t.cos, cs.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
               "upload cos table");
     hip_check(hipMemcpyAsync(t.sin, sn.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
               "upload sin table");
+    hip_check(hipMemcpyAsync(t.alist, alist.data(), sizeof(int) * na, hipMemcpyHostToDevice,
+                             c->stream),
+              "upload angle list");
     hip_check(hipStreamSynchronize(c->stream), "trig upload");
     return c->trig[std::move(key)] = t;
 }
@@ -145,15 +159,14 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
                  float* d_sino, float* d_mm) {
     unsigned* keys = ensure<unsigned>(c, c->keys, sizeof(unsigned) * 2 * (size_t)nz);
     const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
-    // per z-block h x (w + kPadX) pixels of 2 float4 (zero pad included)
-    const size_t lay = sizeof(float4) * 2 * rzb * (size_t)(w + kPadX) * h;
-    float4* row = ensure<float4>(c, c->row, lay);
-    float4* col = ensure<float4>(c, c->col, lay);
+    float4* row = ensure<float4>(c, c->row, rzb * radon_zbytes(w, h));
+    float4* col = ensure<float4>(c, c->col, rzb * radon_zbytes(w, h));
     hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
     hip_check(launch_normalise(d_vol, w, h, nz, d_mm, row, col, c->stream), "normalise");
-    hip_check(launch_radon(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin, d_sino, c->stream),
+    hip_check(launch_radon(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
+                           c->cur.n_row, d_sino, c->stream),
               "radon");
 }
 
@@ -377,6 +390,7 @@ int ct_hip_clear_workspace(ct_hip_ctx* c) {
         for (auto& kv : c->trig) {
             dfree(c, kv.second.cos);
             dfree(c, kv.second.sin);
+            dfree(c, kv.second.alist);
         }
         c->trig.clear();
         c->cur = ct_hip_ctx::Trig();

@@ -20,6 +20,11 @@ constexpr int kRZ = 8;
 // zero pixels after each row (row layout) and columns after the last column
 // (column layout) of the Radon stage's normalised copies
 constexpr int kPadX = 4;
+// bytes of one z-block of either normalised copy: (w + kPadX) x h pixels of
+// 32 bytes (8 slices)
+__host__ __device__ inline size_t radon_zbytes(int w, int h) {
+    return (size_t)(w + kPadX) * h * 32;
+}
 
 // Largest FFT length of the LDS row filter (bins <= 1448).
 constexpr int kMaxFFT = 4096;
@@ -48,13 +53,15 @@ hipError_t launch_minmax(const float* d_vol, size_t slice_elems, int nz, unsigne
                          hipStream_t s);
 // writes mm[2*z] = min, mm[2*z+1] = max as floats
 hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hipStream_t s);
-// w = (raw == 0) ? 0 : (raw - min) * (1/span); z-blocked row layout [zb][h][w]
-// and column layout [zb][w][h], kRZ = 8 slices per pixel (2 float4).
+// w = (raw == 0) ? 0 : (raw - min) * (1/span); z-blocked row copy
+// [zb][h][w + kPadX] and column copy [zb][w + kPadX][h], kRZ = 8 slices per
+// pixel (2 float4), kPadX zero pixels past each row / column.
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, hipStream_t s);
+// alist: angle indices, the n_row row-walk angles (|sin| <= |cos|) first
 hipError_t launch_radon(const float4* d_row, const float4* d_col, int w, int h, int nz, int na,
-                        int bins, const float* d_cos, const float* d_sin, float* d_sino,
-                        hipStream_t s);
+                        int bins, const float* d_cos, const float* d_sin, const int* d_alist,
+                        int n_row, float* d_sino, hipStream_t s);
 
 // ---- row filter ----------------------------------------------------------
 struct FiltArgs {

@@ -74,6 +74,8 @@ struct ct_hip_ctx {
     struct Trig {
         float* cos = nullptr;
         float* sin = nullptr;
+        int* alist = nullptr;  // angle indices: row-walk angles (|sin| <= |cos|) first
+        int n_row = 0;         // how many of them walk rows
     };
     std::map<std::vector<uint32_t>, Trig> trig;  // keyed by the angle list's bit pattern
     Trig cur;                                     // table of the current plain call

@@ -82,13 +82,13 @@ __global__ void minmax_decode_kernel(const unsigned* keys, int nz, float* mm) {
 
 // block (32,8) handles a 32x32 pixel tile of one Radon z-block (kRZ = 8
 // slices): reads the slices, normalises (SerialBackend.cpp:78-80,104,106),
-// writes the row layout directly and the column layout through an LDS
+// writes the row copy directly and the column copy through an LDS
 // transpose. A pixel is 32 bytes (the 8 slices as 2 float4):
-//   row layout    rowL[(((zb*h) + y)*(w + kPadX) + x)*2 + {0,1}]
-//   column layout colL[(((zb*(w + kPadX)) + x)*h + y)*2 + {0,1}]
+//   row copy     byte ((zb*h + y)*(w + kPadX) + x)*32
+//   column copy  byte ((zb*(w + kPadX) + x)*h + y)*32
 // x runs to w + kPadX - 1: the kPadX pixels past each row (columns past the
-// last column) are written as zeros, so a ray's fixed-size chunk of
-// candidates may run past the edge of the image and read only zeros.
+// last column) are zeros, so a ray's fixed-size chunk of candidates may run
+// past the edge of the image and read only zeros.
 __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict__ vol, int w,
                                                         int h, int nz,
                                                         const float* __restrict__ mm,
@@ -193,6 +193,7 @@ __device__ __forceinline__ void row_setup(const RayGeom& g, int y, int& x0, int&
     }
 }
 
+// Loads of one chunk, the pixels at byte offsets off + k*xstep, k < U.
 template <int U>
 __device__ __forceinline__ void chunk_load(const char* __restrict__ p, unsigned off,
                                            unsigned xstep, float4 (&v)[U][2]) {
@@ -233,8 +234,8 @@ __device__ __forceinline__ void chunk_add(const RayGeom& g, int x, float sy, con
 // first candidate. All loads of a chunk are issued before its arithmetic.
 template <bool STEEP, int U>
 __device__ __forceinline__ void ray_walk(const RayGeom& g, const char* __restrict__ p,
-                                         unsigned ystep,
-                                         unsigned xstep, int y0, int y1, float (&acc)[kRZ]) {
+                                         unsigned ystep, unsigned xstep, int y0, int y1,
+                                         float (&acc)[kRZ]) {
     for (int y = y0; y <= y1; ++y) {
         int x0, x1;
         float sy;
@@ -248,17 +249,12 @@ __device__ __forceinline__ void ray_walk(const RayGeom& g, const char* __restric
     }
 }
 
-__global__ __launch_bounds__(256) void radon_ray_kernel(
-    const float4* __restrict__ rowL, const float4* __restrict__ colL, int w, int h, int nz,
-    int na, int bins, const float* __restrict__ cos_t, const float* __restrict__ sin_t,
-    float* __restrict__ sino, int zb0) {
-    const int i = blockIdx.x * 256 + threadIdx.x;  // bin owned by this lane
-    const int a = blockIdx.y;
-    const int zb = zb0 + blockIdx.z;
-    if (i >= bins) return;
+// Lane geometry of bin i at angle (c, sn); also the candidate rows y0..y1.
+__device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float c, float sn,
+                                           int& y0, int& y1) {
     RayGeom g;
-    g.c = cos_t[a];
-    g.sn = sin_t[a];
+    g.c = c;
+    g.sn = sn;
     g.cx = (float)(w - 1) * 0.5f;
     g.cy = (float)(h - 1) * 0.5f;
     g.dc = (float)(bins / 2);
@@ -266,23 +262,19 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
     g.i = i;
     g.im1 = (i >= 1) ? i - 1 : -0x7fffffff;
     g.keep = (i == bins - 1) ? 0.0f : 1.0f;  // SerialBackend.cpp:113-116
-    const bool col = fabsf(g.sn) > fabsf(g.c);
-    // one z-block's copy; byte offsets within it fit 32 bits
-    const size_t wp = (size_t)(w + kPadX);
-    const char* p = reinterpret_cast<const char*>((col ? colL : rowL) + (size_t)zb * h * wp * 2);
-
     g.slo = (float)(i - 1) - kDelta - g.dc;  // s - dc lower bound
     g.shi = (float)(i + 1) + kDelta - g.dc;  // s - dc upper bound
-    const float ac = fabsf(g.c), as = fabsf(g.sn);
+    const float ac = fabsf(c), as = fabsf(sn);
 
     // rows that can hold a candidate: (y-cy)*sn in [slo - cx|c|, shi + cx|c|]
-    int y0 = 0, y1 = h - 1;
+    y0 = 0;
+    y1 = h - 1;
     if (as > 1e-20f) {
         const float lo = (g.slo - g.cx * ac) / as;
         const float hi = (g.shi + g.cx * ac) / as;
         float ya, yb;
-        if (g.sn > 0.0f) { ya = g.cy + lo; yb = g.cy + hi; }
-        else             { ya = g.cy - hi; yb = g.cy - lo; }
+        if (sn > 0.0f) { ya = g.cy + lo; yb = g.cy + hi; }
+        else           { ya = g.cy - hi; yb = g.cy - lo; }
         y0 = max(0, (int)floorf(fmaxf(ya, -2.0f)) - 1);
         y1 = min(h - 1, (int)ceilf(fminf(yb, (float)h + 1.0f)) + 1);
     }
@@ -294,26 +286,148 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
     // Steeper rays compute it per row from sy, whose error then scales with
     // the distance to the strip and vanishes where candidates exist.
     g.has_c = ac > 1e-20f;
-    g.inv_c = g.has_c ? 1.0f / g.c : 0.0f;
+    g.inv_c = g.has_c ? 1.0f / c : 0.0f;
     g.xa0 = 0.0f;
     g.dxa = 0.0f;
-    if (!col) {
-        const float e0 = (g.slo + g.cy * g.sn) * g.inv_c, e1 = (g.shi + g.cy * g.sn) * g.inv_c;
+    if (!(as > ac)) {
+        const float e0 = (g.slo + g.cy * sn) * g.inv_c, e1 = (g.shi + g.cy * sn) * g.inv_c;
         g.xa0 = g.cx + fminf(e0, e1);
-        g.dxa = -g.sn * g.inv_c;
+        g.dxa = -sn * g.inv_c;
+    }
+    return g;
+}
+
+// A bin no floor(s) equals: a lane holding it adds only +0.
+constexpr int kNoBin = -0x7fffffff - 1;
+
+// Rays of angles with |sin| > |cos|: each lane walks its rows with direct
+// loads. Lanes whose ray enters through the top edge (first candidate row
+// 0) are on one common row at every step, spaced 1/|cos| pixels apart: they
+// read the row-major copy. The others enter through a side edge and are on
+// consecutive rows of one common column at every step: they read the
+// column-major copy. Either way a wave's loads are close to contiguous.
+__global__ __launch_bounds__(256) void radon_ray_kernel(
+    const float4* __restrict__ rowL, const float4* __restrict__ colL, int w, int h, int nz,
+    int na, int bins, const float* __restrict__ cos_t, const float* __restrict__ sin_t,
+    const int* __restrict__ alist, float* __restrict__ sino, int zb0) {
+    const int i = blockIdx.x * 256 + threadIdx.x;  // bin owned by this lane
+    const int a = alist[blockIdx.y];
+    const int zb = zb0 + blockIdx.z;
+    if (i >= bins) return;
+    int y0, y1;
+    const RayGeom g = ray_geom(i, bins, w, h, cos_t[a], sin_t[a], y0, y1);
+    // one z-block's copy; byte offsets within it fit 32 bits
+    const unsigned wp = (unsigned)(w + kPadX);
+    const bool top = (y0 == 0);
+    const char* p = reinterpret_cast<const char*>(top ? rowL : colL) + (size_t)zb * radon_zbytes(w, h);
+    // row copy: x steps 32 bytes, y steps 32 (w + kPadX);
+    // column copy: x steps 32 h bytes, y steps 32
+    const unsigned ystep = top ? 32u * wp : 32u;
+    const unsigned xstep = top ? 32u : 32u * (unsigned)h;
+    float acc[kRZ];
+#pragma unroll
+    for (int k = 0; k < kRZ; ++k) acc[k] = 0.0f;
+    ray_walk<true, 3>(g, p, ystep, xstep, y0, y1, acc);
+#pragma unroll
+    for (int k = 0; k < kRZ; ++k) {
+        const int z = zb * kRZ + k;
+        if (z < nz) sino[((size_t)z * na + a) * bins + i] = acc[k];
+    }
+}
+
+// Rays of angles with |sin| <= |cos|: every lane of a block (256 bins of one
+// angle and z-block) walks all rows top to bottom, at most 3 candidates per
+// row. The block stages RR rows at a time into LDS, each as a window of kRowW
+// pixels starting at the leftmost lane's first candidate. A lane's first
+// candidate x0(y) is a composition of monotone float operations of its bin,
+// so the window start is that of the extreme bin (the lowest one for
+// cos > 0, the highest for cos < 0), and every lane's chunk lies within
+// kRowW pixels of it: lanes 255 bins apart are at most 255/|cos| <= 360.7
+// pixels apart, plus 3 pixels of chunk. Rows outside a lane's candidate rows
+// hold no candidates of its bin and add +0.
+constexpr int kRowW = 368;
+constexpr int kRadonRows = 2;  // rows staged per step
+
+template <int RR>
+__global__ __launch_bounds__(256) void radon_rows_kernel(
+    const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
+    const float* __restrict__ cos_t, const float* __restrict__ sin_t,
+    const int* __restrict__ alist, float* __restrict__ sino) {
+    __shared__ float4 t0[RR][kRowW];  // slices 0-3 of the staged pixels
+    __shared__ float4 t1[RR][kRowW];  // slices 4-7
+    __shared__ int xw_s[2][RR];       // window starts: staged rows, next rows
+    const int a = alist[blockIdx.y];
+    const int zb = blockIdx.z;
+    const int b0 = blockIdx.x * 256;
+    const int i = b0 + threadIdx.x;
+    const bool live = i < bins;
+    const float c = cos_t[a], sn = sin_t[a];
+    int y0, y1;
+    RayGeom g = ray_geom(live ? i : bins - 1, bins, w, h, c, sn, y0, y1);
+    if (!live) g.i = g.im1 = kNoBin;  // walks along, adds nothing
+    const size_t wp = (size_t)(w + kPadX);
+    const float4* __restrict__ slab = rowL + (size_t)zb * h * wp * 2;
+    // the extreme bin's geometry, held by the first RR lanes only: lane r
+    // computes the window start of row ys + r of the next step
+    const int ie = c > 0.0f ? b0 : min(b0 + 255, bins - 1);
+    RayGeom ge;
+    if (threadIdx.x < RR) {
+        int ey0, ey1;
+        ge = ray_geom(ie, bins, w, h, c, sn, ey0, ey1);
+        int xd;
+        float sd;
+        row_setup<false>(ge, threadIdx.x, xw_s[0][threadIdx.x], xd, sd);
     }
 
     float acc[kRZ];
 #pragma unroll
     for (int k = 0; k < kRZ; ++k) acc[k] = 0.0f;
-    if (!col)  // row layout: x steps 32 bytes, y steps 32 (w + kPadX)
-        ray_walk<false, 3>(g, p, 32u * (unsigned)wp, 32u, y0, y1, acc);
-    else       // column layout: x steps 32 h bytes, y steps 32
-        ray_walk<true, 3>(g, p, 32u, 32u * (unsigned)h, y0, y1, acc);
+    int buf = 0;
+    for (int ys = 0; ys < h; ys += RR, buf ^= 1) {
+        __syncthreads();  // the previous rows are consumed; xw_s[buf] is set
+        if (threadIdx.x < RR) {  // window starts of the next step's rows
+            int xd;
+            float sd;
+            row_setup<false>(ge, ys + RR + threadIdx.x, xw_s[buf ^ 1][threadIdx.x], xd, sd);
+        }
 #pragma unroll
-    for (int k = 0; k < kRZ; ++k) {
-        const int z = zb * kRZ + k;
-        if (z < nz) sino[((size_t)z * na + a) * bins + i] = acc[k];
+        for (int r = 0; r < RR; ++r) {
+            const int y = ys + r;
+            const int xw = xw_s[buf][r];
+            for (int j = threadIdx.x; j < kRowW; j += 256) {
+                const int x = xw + j;
+                float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
+                if (y < h && x < w + kPadX) {  // pixels past the pad are zero too
+                    const float4* q = slab + ((size_t)y * wp + x) * 2;
+                    v0 = q[0];
+                    v1 = q[1];
+                }
+                t0[r][j] = v0;
+                t1[r][j] = v1;
+            }
+        }
+        __syncthreads();
+        for (int r = 0; r < RR && ys + r < h; ++r) {
+            const int y = ys + r;
+            int x0, x1;
+            float sy;
+            row_setup<false>(g, y, x0, x1, sy);
+            const int d = x0 - xw_s[buf][r];
+            float4 v[3][2];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                v[k][0] = t0[r][d + k];
+                v[k][1] = t1[r][d + k];
+            }
+            chunk_add<3>(g, x0, sy, v, acc);
+        }
+    }
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < kRZ; ++k) {
+            const int z = zb * kRZ + k;
+            if (z < nz) sino[((size_t)z * na + a) * bins + i] = acc[k];
+        }
     }
 }
 
@@ -347,20 +461,34 @@ hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const floa
 }
 
 hipError_t launch_radon(const float4* d_row, const float4* d_col, int w, int h, int nz, int na,
-                        int bins, const float* d_cos, const float* d_sin, float* d_sino,
-                        hipStream_t s) {
+                        int bins, const float* d_cos, const float* d_sin, const int* d_alist,
+                        int n_row, float* d_sino, hipStream_t s) {
     const int nzb = (nz + kRZ - 1) / kRZ;
-    int per = nzb;  // z-blocks per launch
-    if (const char* env = std::getenv("CT_HIP_RADON_ZB_PER_LAUNCH")) {
-        const int v = std::atoi(env);
-        if (v >= 1) per = v;
-    }
-    for (int zb0 = 0; zb0 < nzb; zb0 += per) {
-        dim3 grid((bins + 255) / 256, na, std::min(per, nzb - zb0));
-        hipLaunchKernelGGL(radon_ray_kernel, grid, dim3(256), 0, s, d_row, d_col, w, h, nz, na,
-                           bins, d_cos, d_sin, d_sino, zb0);
+    const int bb = (bins + 255) / 256;
+    if (n_row > 0) {
+        hipLaunchKernelGGL(radon_rows_kernel<kRadonRows>, dim3(bb, n_row, nzb), dim3(256), 0, s,
+                           d_row, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+    }
+    if (na - n_row > 0) {
+        // About 1024 blocks per launch (a few z-blocks): the blocks of one
+        // launch start together and sweep the same copy in step, so most of
+        // a z-block's pixels come from HBM once per launch rather than once
+        // per angle.
+        const int zbb = bb * (na - n_row);
+        int per = std::max(1, (1024 + zbb - 1) / zbb);
+        if (const char* env = std::getenv("CT_HIP_RADON_STEEP_ZB")) {
+            const int v = std::atoi(env);
+            if (v >= 1) per = v;
+        }
+        for (int zb0 = 0; zb0 < nzb; zb0 += per) {
+            hipLaunchKernelGGL(radon_ray_kernel, dim3(bb, na - n_row, std::min(per, nzb - zb0)),
+                               dim3(256), 0, s, d_row, d_col, w, h, nz, na, bins, d_cos, d_sin,
+                               d_alist + n_row, d_sino, zb0);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
     }
     return hipSuccess;
 }
