@@ -1,7 +1,8 @@
 """Benchmark of the FBP hot path on one MI355X.
 
 A step is one reconstructVolume over device-resident input: per-slice
-min/max + normalisation, ray-driven Radon (720 angles x 512 bins), the
+min/max + normalisation, ray-driven Radon (720 angles x 512 bins; an
+LDS-staged row walk and a direct walk, each timed with HIP events), the
 Shepp-Logan FFT row filter (sq 725, nfft 2048) and voxel-driven
 backprojection with denormalisation, for a 512^3 phantom of 24 seeded
 ellipsoids (the plain-FP32 single-GPU configuration). value = voxel-updates/s
@@ -26,7 +27,17 @@ sys.path.insert(0, ROOT)
 METRIC = "FBP voxel-updates/s + p50/p99 latency, served vs plain HIP, relL2≤1e-4 vs FP64"
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector and matrix alike)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
-FLOP_PER_UPDATE = 9        # t (3), u (1), frac (1), lerp (3), accumulate (1)
+FLOP_PER_UPDATE = 9        # backprojection: t (3), u (1), frac (1), lerp (3), accumulate (1)
+# Radon: per (pixel, angle, slice) the two deposits v*(1-frac), v*frac and their
+# two accumulations (SerialBackend.cpp:113-120); the geometry (s, floor, frac)
+# is per (pixel, angle) and shared by the slices of a z-block.
+FLOP_PER_SAMPLE = 4
+# HBM bytes per launch of each kernel at the 512^3 x 720 workload, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over one step, each scaled
+# by the counter's reading for a kernel of known traffic (minmax_kernel reads
+# 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
+# profile summaries list the raw counts and the command.
+TRAFFIC_BYTES = {}
 
 
 def log(msg):
@@ -55,6 +66,48 @@ def cpu_baseline(vol_host: np.ndarray, na: int, ftype: int, target_s: float):
             "sample": f"OpenMPBackend::reconstructVolume restatement ({O.ompbase().isa}), "
                       f"{n}x{n}x{nz} slices of the same phantom, {na} angles, "
                       f"{dt:.2f} s wall"}
+
+
+def row_walk_angles(na: int) -> int:
+    """Angles of the Serial set with |sin| <= |cos| (float32 trig, as the host tables)."""
+    a = np.arange(na, dtype=np.float32) * np.float32(180.0 / na)
+    th = (a * np.float32(3.14159265359)) / np.float32(180.0)
+    c = np.cos(th.astype(np.float64)).astype(np.float32)
+    s = np.sin(th.astype(np.float64)).astype(np.float32)
+    return int(np.count_nonzero(~(np.abs(s) > np.abs(c))))
+
+
+def dominant_roofline(n, na, stage, radon_k, dir_launches):
+    """Roofline entry of the kernel with the largest time per step.
+
+    Candidates: the Radon direct walk (radon_ray_kernel, several launches),
+    the Radon row walk (radon_rows_kernel), the backprojection (bp_kernel)
+    and the FFT row filter (row_filter_fft_kernel). achieved = algorithmic
+    FLOP per launch / average launch time (HIP events on the context stream).
+    """
+    n_row = row_walk_angles(na)
+    samples = float(n) * n * n  # pixels x slices per angle
+    cands = {
+        "radon_ray_kernel": (radon_k[1], max(dir_launches, 1),
+                             FLOP_PER_SAMPLE * samples * (na - n_row),
+                             "Radon direct walk (angles with |sin| > |cos|)"),
+        "radon_rows_kernel": (radon_k[0], 1, FLOP_PER_SAMPLE * samples * n_row,
+                              "Radon LDS row walk (angles with |sin| <= |cos|)"),
+        "bp_kernel": (stage[2], 1, FLOP_PER_UPDATE * samples * na,
+                      "voxel-driven backprojection"),
+    }
+    name = max(cands, key=lambda k: cands[k][0])
+    t_ms, launches, flop, what = cands[name]
+    per_launch_ms = t_ms / launches
+    achieved = flop / launches / (per_launch_ms / 1e3) / 1e12
+    tb = TRAFFIC_BYTES.get(name)
+    return {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": tb, "kernel": name,
+            "launches_per_step": launches, "avg_launch_ms": per_launch_ms,
+            "note": f"{what}; FP32 VALU class (no MFMA work on this path); "
+                    f"{'4 FLOP per pixel-angle-slice' if name.startswith('radon') else '9 FLOP per voxel-update'}"
+                    "; 157.3 TF is MI355X's FP32 peak (packed FP32 counted), the kernels are "
+                    "built without packed FP32 ops"}
 
 
 def precision_form(ctx, d_sino, n, na, sample_z, host_sinos_sample):
@@ -278,21 +331,26 @@ def main():
         step()
     ctx.synchronize()
     stage = np.zeros(4)
+    radon_k = np.zeros(2)
     log(f"[bench] timing {args.steps} steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
         stage += ctx.timings()
+        r_row, r_dir, dir_launches = ctx.radon_timings()
+        radon_k += (r_row, r_dir)
     ctx.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     stage /= args.steps
+    radon_k /= args.steps
     updates = float(nvox) * na
     value = updates / (ms / 1e3)
-    bp_ms = stage[2]
-    bp_tflops = FLOP_PER_UPDATE * updates / (bp_ms / 1e3) / 1e12
-    log(f"[bench] {ms:.2f} ms/step; stages radon {stage[0]:.2f} filter {stage[1]:.2f} "
+    log(f"[bench] {ms:.2f} ms/step; stages radon {stage[0]:.2f} (row walk {radon_k[0]:.2f}, "
+        f"direct walk {radon_k[1]:.2f} in {dir_launches} launches) filter {stage[1]:.2f} "
         f"bp {stage[2]:.2f} ms")
+    roofline = dominant_roofline(n, na, stage, radon_k, dir_launches)
+    log(f"[bench] roofline {roofline}")
 
     extra = {}
     vol_host = None
@@ -347,11 +405,9 @@ def main():
         "stages_ms": {"normalise_radon": stage[0], "row_filter": stage[1],
                       "backprojection": stage[2]},
         "sinogram_samples_per_s": float(n) * na * n / (ms / 1e3),
-        "roofline": {"bound": "mfma", "achieved": bp_tflops, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": bp_tflops / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "bp_kernel (backprojection)",
-                     "note": "FP32 VALU-bound; 9 algorithmic FLOP per voxel-update; "
-                             "157.3 TF is MI355X's FP32 peak (vector and matrix alike)"},
+        "stages_ms_radon": {"row_walk": radon_k[0], "direct_walk": radon_k[1],
+                            "direct_walk_launches": dir_launches},
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -127,6 +127,8 @@ def load_library(path: str = LIB_PATH):
         "ct_hip_noisy_log_dev": (i, [ctx, _vp, sz, ctypes.c_float, ctypes.c_float, ctypes.c_uint64]),
         "ct_hip_filter_sinograms": (i, [ctx, _f32p, i, i, i, i, _f32p]),
         "ct_hip_last_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double)]),
+        "ct_hip_last_radon_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -153,6 +155,7 @@ def exported_symbols() -> List[str]:
         "ct_hip_server_download", "ct_hip_server_release", "ct_hip_server_stats",
         "ct_hip_server_last_error", "ct_hip_radon_dev", "ct_hip_noisy_log_dev",
         "ct_hip_last_timings",
+        "ct_hip_last_radon_timings",
     ]
 
 
@@ -264,6 +267,13 @@ class Context:
         t = (ctypes.c_double * 4)()
         self._call("ct_hip_last_timings", self.handle, t)
         return np.array(t[:], np.float64)
+
+    def radon_timings(self):
+        """(row-walk ms, direct-walk ms, direct-walk launches) of the last Radon stage."""
+        t = (ctypes.c_double * 2)()
+        n = ctypes.c_int(0)
+        self._call("ct_hip_last_radon_timings", self.handle, t, ctypes.byref(n))
+        return float(t[0]), float(t[1]), int(n.value)
 
     def design_filter(self, n: int, ftype: int) -> np.ndarray:
         out = np.empty(n // 2 + 1, np.float32)

@@ -165,9 +165,17 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
     hip_check(launch_normalise(d_vol, w, h, nz, d_mm, row, col, c->stream), "normalise");
-    hip_check(launch_radon(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
-                           c->cur.n_row, d_sino, c->stream),
-              "radon");
+    hip_check(hipEventRecord(c->evr[0], c->stream), "event");
+    hip_check(launch_radon_rows(row, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
+                                c->cur.n_row, d_sino, c->stream),
+              "radon rows");
+    hip_check(hipEventRecord(c->evr[1], c->stream), "event");
+    hip_check(launch_radon_direct(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin,
+                                  c->cur.alist + c->cur.n_row, na - c->cur.n_row, d_sino,
+                                  c->stream, &c->radon_direct_launches),
+              "radon direct");
+    hip_check(hipEventRecord(c->evr[2], c->stream), "event");
+    c->radon_timed = true;
 }
 
 const ct_hip_ctx::Toep& get_toeplitz(ct_hip_ctx* c, const Geom& g, int type) {
@@ -240,7 +248,14 @@ void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int
 void read_timings(ct_hip_ctx* c, bool with_radon) {
     hip_check(hipEventSynchronize(c->ev[3]), "event sync");
     float t01 = 0, t12 = 0, t23 = 0;
-    if (with_radon) hip_check(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]), "elapsed");
+    if (with_radon) {
+        hip_check(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]), "elapsed");
+        float r0 = 0, r1 = 0;
+        hip_check(hipEventElapsedTime(&r0, c->evr[0], c->evr[1]), "elapsed");
+        hip_check(hipEventElapsedTime(&r1, c->evr[1], c->evr[2]), "elapsed");
+        c->radon_ms[0] = r0;
+        c->radon_ms[1] = r1;
+    }
     hip_check(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]), "elapsed");
     hip_check(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]), "elapsed");
     c->timings[0] = t01;
@@ -351,6 +366,12 @@ int ct_hip_create(int device, size_t hbm_budget_bytes, ct_hip_ctx** out) {
         delete c;
         return CT_ERR_HIP;
     }
+    for (auto& e : c->evr) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            ct_hip_destroy(c);
+            return CT_ERR_HIP;
+        }
+    }
     for (auto& e : c->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             ct_hip_destroy(c);
@@ -372,6 +393,8 @@ void ct_hip_destroy(ct_hip_ctx* c) {
     for (auto& kv : c->sizes) (void)hipFree(kv.first);
     c->sizes.clear();
     for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->evr)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -762,6 +785,15 @@ int ct_hip_filter_sinograms(ct_hip_ctx* c, const float* sinos, int nz, int na, i
 int ct_hip_last_timings(const ct_hip_ctx* c, double* ms4) {
     if (!c || !ms4) return CT_ERR_INVALID;
     for (int i = 0; i < 4; ++i) ms4[i] = c->timings[i];
+    return CT_OK;
+}
+
+int ct_hip_last_radon_timings(const ct_hip_ctx* c, double* ms2, int* direct_launches) {
+    if (!c || !ms2) return CT_ERR_INVALID;
+    if (!c->radon_timed) return CT_ERR_INVALID;
+    ms2[0] = c->radon_ms[0];
+    ms2[1] = c->radon_ms[1];
+    if (direct_launches) *direct_launches = c->radon_direct_launches;
     return CT_OK;
 }
 

@@ -81,6 +81,11 @@ struct ct_hip_ctx {
     Trig cur;                                     // table of the current plain call
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double timings[4] = {0, 0, 0, 0};
+    // Radon stage breakdown: events around the row walk and the direct walk
+    hipEvent_t evr[3] = {nullptr, nullptr, nullptr};
+    double radon_ms[2] = {0, 0};
+    int radon_direct_launches = 0;
+    bool radon_timed = false;
     int filter_backend = 0;  // CT_FILTER_BACKEND_*
     int bp_config = 0;       // backprojection tile configuration (kBpConfigs index)
     bool bp_config_fixed = false;  // set by ct_hip_set_bp_config; else tuned per (N, na)

@@ -460,35 +460,42 @@ hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const floa
     return hipGetLastError();
 }
 
-hipError_t launch_radon(const float4* d_row, const float4* d_col, int w, int h, int nz, int na,
-                        int bins, const float* d_cos, const float* d_sin, const int* d_alist,
-                        int n_row, float* d_sino, hipStream_t s) {
+hipError_t launch_radon_rows(const float4* d_row, int w, int h, int nz, int na, int bins,
+                             const float* d_cos, const float* d_sin, const int* d_alist,
+                             int n_row, float* d_sino, hipStream_t s) {
+    if (n_row <= 0) return hipSuccess;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int bb = (bins + 255) / 256;
-    if (n_row > 0) {
-        hipLaunchKernelGGL(radon_rows_kernel<kRadonRows>, dim3(bb, n_row, nzb), dim3(256), 0, s,
-                           d_row, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino);
+    hipLaunchKernelGGL(radon_rows_kernel<kRadonRows>, dim3(bb, n_row, nzb), dim3(256), 0, s,
+                       d_row, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino);
+    return hipGetLastError();
+}
+
+hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, int h, int nz,
+                               int na, int bins, const float* d_cos, const float* d_sin,
+                               const int* d_alist, int n_dir, float* d_sino, hipStream_t s,
+                               int* launches) {
+    if (launches) *launches = 0;
+    if (n_dir <= 0) return hipSuccess;
+    const int nzb = (nz + kRZ - 1) / kRZ;
+    const int bb = (bins + 255) / 256;
+    // At least ~720 blocks per launch, as few z-blocks as that allows: the
+    // blocks of one launch start together and sweep the same copy in step, so
+    // most of a z-block's pixels come from HBM once per launch rather than
+    // once per angle.
+    const int zbb = bb * n_dir;
+    int per = std::max(1, (720 + zbb - 1) / zbb);
+    if (const char* env = std::getenv("CT_HIP_RADON_STEEP_ZB")) {
+        const int v = std::atoi(env);
+        if (v >= 1) per = v;
+    }
+    for (int zb0 = 0; zb0 < nzb; zb0 += per) {
+        hipLaunchKernelGGL(radon_ray_kernel, dim3(bb, n_dir, std::min(per, nzb - zb0)), dim3(256),
+                           0, s, d_row, d_col, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino,
+                           zb0);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-    }
-    if (na - n_row > 0) {
-        // About 1024 blocks per launch (a few z-blocks): the blocks of one
-        // launch start together and sweep the same copy in step, so most of
-        // a z-block's pixels come from HBM once per launch rather than once
-        // per angle.
-        const int zbb = bb * (na - n_row);
-        int per = std::max(1, (1024 + zbb - 1) / zbb);
-        if (const char* env = std::getenv("CT_HIP_RADON_STEEP_ZB")) {
-            const int v = std::atoi(env);
-            if (v >= 1) per = v;
-        }
-        for (int zb0 = 0; zb0 < nzb; zb0 += per) {
-            hipLaunchKernelGGL(radon_ray_kernel, dim3(bb, na - n_row, std::min(per, nzb - zb0)),
-                               dim3(256), 0, s, d_row, d_col, w, h, nz, na, bins, d_cos, d_sin,
-                               d_alist + n_row, d_sino, zb0);
-            const hipError_t e = hipGetLastError();
-            if (e != hipSuccess) return e;
-        }
+        if (launches) ++*launches;
     }
     return hipSuccess;
 }
