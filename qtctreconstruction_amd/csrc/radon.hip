@@ -236,6 +236,9 @@ template <bool STEEP, int U>
 __device__ __forceinline__ void ray_walk(const RayGeom& g, const char* __restrict__ p,
                                          unsigned ystep, unsigned xstep, int y0, int y1,
                                          float (&acc)[kRZ]) {
+    // a chunk starts at x <= x1 <= w - 1 and reaches x + U - 1 <= w + U - 2,
+    // which must stay within the zero pad (pixels w .. w + kPadX - 1)
+    static_assert(U - 2 <= kPadX - 1, "chunk reaches past the zero pad");
     for (int y = y0; y <= y1; ++y) {
         int x0, x1;
         float sy;
@@ -306,6 +309,7 @@ constexpr int kNoBin = -0x7fffffff - 1;
 // read the row-major copy. The others enter through a side edge and are on
 // consecutive rows of one common column at every step: they read the
 // column-major copy. Either way a wave's loads are close to contiguous.
+template <int U>
 __global__ __launch_bounds__(256) void radon_ray_kernel(
     const float4* __restrict__ rowL, const float4* __restrict__ colL, int w, int h, int nz,
     int na, int bins, const float* __restrict__ cos_t, const float* __restrict__ sin_t,
@@ -327,7 +331,7 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
     float acc[kRZ];
 #pragma unroll
     for (int k = 0; k < kRZ; ++k) acc[k] = 0.0f;
-    ray_walk<true, 3>(g, p, ystep, xstep, y0, y1, acc);
+    ray_walk<true, U>(g, p, ystep, xstep, y0, y1, acc);
 #pragma unroll
     for (int k = 0; k < kRZ; ++k) {
         const int z = zb * kRZ + k;
@@ -490,9 +494,10 @@ hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, 
         if (v >= 1) per = v;
     }
     for (int zb0 = 0; zb0 < nzb; zb0 += per) {
-        hipLaunchKernelGGL(radon_ray_kernel, dim3(bb, n_dir, std::min(per, nzb - zb0)), dim3(256),
-                           0, s, d_row, d_col, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino,
-                           zb0);
+        // chunks of 4 candidates (8 loads in flight per lane)
+        hipLaunchKernelGGL(radon_ray_kernel<4>, dim3(bb, n_dir, std::min(per, nzb - zb0)),
+                           dim3(256), 0, s, d_row, d_col, w, h, nz, na, bins, d_cos, d_sin,
+                           d_alist, d_sino, zb0);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (launches) ++*launches;
