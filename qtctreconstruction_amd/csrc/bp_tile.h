@@ -17,8 +17,8 @@
 // angle, the detector window the tile can touch (F bins from
 // floor(min corner u) - 1; u is monotone in x and y, so the corners bound it)
 // as (f, f[i+1]-f[i]) float4 pairs: both interpolation operands for 4 slices
-// in two 16-byte LDS reads. The trig tables are read with wave-uniform
-// indices, i.e. through the scalar cache.
+// in two 16-byte LDS reads. Each stage also stages its angles' (cos, sin),
+// read in the angle loop as one broadcast LDS value.
 #pragma once
 
 #include "ct_internal.h"
@@ -32,8 +32,18 @@ template <int T, int A>
 struct BpSmem {
     float4 f[A][bp_window(T)];
     float4 d[A][bp_window(T)];
+    float2 cs[A];  // (cos, sin) of the stage's angles
     int lo[A];
 };
+
+// Global-memory views of the argument pointers. The persistent kernel reads
+// its arguments from a work list in memory, where the compiler only sees
+// generic pointers; naming the address space gives both kernels the same
+// global (not flat) loads and stores.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const f4v __attribute__((address_space(1))) g_cfloat4;
+typedef float __attribute__((address_space(1))) g_float;
+typedef const float __attribute__((address_space(1))) g_cfloat;
 
 template <int T, int A, bool CHECK>
 __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
@@ -57,12 +67,15 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
     for (int i = 0; i < V; ++i)
 #pragma unroll
         for (int j = 0; j < V; ++j) acc[i][j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4* __restrict__ fz = p.filt + (size_t)zb * p.na * p.sq;
+    g_cfloat4* __restrict__ fz = (g_cfloat4*)p.filt + (size_t)zb * p.na * p.sq;
+    g_cfloat* __restrict__ cos_t = (g_cfloat*)p.cos_t;
+    g_cfloat* __restrict__ sin_t = (g_cfloat*)p.sin_t;
 
     for (int a0 = 0; a0 < p.na; a0 += A) {
         const int nc = min(A, p.na - a0);
         if (tid < nc) {
-            const float c = p.cos_t[a0 + tid], s = p.sin_t[a0 + tid];
+            const float c = cos_t[a0 + tid], s = sin_t[a0 + tid];
+            sm.cs[tid] = make_float2(c, s);
             const float u00 = (cxa * c - cya * s) + dcb;
             const float u01 = (cxb * c - cya * s) + dcb;
             const float u10 = (cxa * c - cyb * s) + dcb;
@@ -73,16 +86,17 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
         for (int e = tid; e < nc * F; e += 256) {
             const int aa = e / F, j = e - aa * F;
             const int g = sm.lo[aa] + j;
-            const float4* row = fz + (size_t)(a0 + aa) * p.sq;
-            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            const float4 f0 = (g >= 0 && g < p.sq) ? row[g] : z4;
-            const float4 f1 = (g + 1 >= 0 && g + 1 < p.sq) ? row[g + 1] : z4;
-            sm.f[aa][j] = f0;
+            g_cfloat4* row = fz + (size_t)(a0 + aa) * p.sq;
+            const f4v z4 = {0.f, 0.f, 0.f, 0.f};
+            const f4v f0 = (g >= 0 && g < p.sq) ? row[g] : z4;
+            const f4v f1 = (g + 1 >= 0 && g + 1 < p.sq) ? row[g + 1] : z4;
+            sm.f[aa][j] = make_float4(f0.x, f0.y, f0.z, f0.w);
             sm.d[aa][j] = make_float4(f1.x - f0.x, f1.y - f0.y, f1.z - f0.z, f1.w - f0.w);
         }
         __syncthreads();
         for (int aa = 0; aa < nc; ++aa) {
-            const float c = p.cos_t[a0 + aa], s = p.sin_t[a0 + aa];
+            const float2 cs = sm.cs[aa];
+            const float c = cs.x, s = cs.y;
             const int lo = sm.lo[aa];
             float xc[V], ys[V];
 #pragma unroll
@@ -121,8 +135,9 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
         lo_z[k] = 0.0f;
         eff_z[k] = 1.0f;
         if (p.mm && z < p.nz) {
-            const float span = p.mm[2 * z + 1] - p.mm[2 * z];
-            lo_z[k] = p.mm[2 * z];
+            g_cfloat* mm = (g_cfloat*)p.mm;
+            const float span = mm[2 * z + 1] - mm[2 * z];
+            lo_z[k] = mm[2 * z];
             eff_z[k] = (span <= 0.0f) ? 1.0f : span;  // SerialBackend.cpp:298-304
         }
     }
@@ -141,7 +156,7 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
                 if (z >= p.nz) break;
                 float v = inside ? v4[k] * p.scale : 0.0f;
                 if (p.mm) v = v * eff_z[k] + lo_z[k];
-                p.out[(size_t)z * ss + (size_t)y * p.N + x] = v;
+                ((g_float*)p.out)[(size_t)z * ss + (size_t)y * p.N + x] = v;
             }
         }
 }
