@@ -299,6 +299,9 @@ def main():
     ap.add_argument("--forms", default="precision,batched,served",
                     help="extra forms to measure: precision,batched,served (or none)")
     ap.add_argument("--serve-jobs", type=int, default=2000)
+    ap.add_argument("--bp-config", type=int, default=-1,
+                    help="fix the backprojection tile configuration (index into the "
+                         "instantiated set) instead of autotuning it on first use")
     ap.add_argument("--calibrate", action="store_true",
                     help="also launch the known-traffic kernels used to scale HBM counters: "
                          "phantom_kernel with 0 ellipsoids writes exactly 4*n^3 bytes, "
@@ -321,6 +324,8 @@ def main():
         ctx.phantom_dev(args.seed, 0, n, n, n, d_vol)
     ctx.phantom_dev(args.seed, args.ellipsoids, n, n, n, d_vol)
     lib = ctx.lib
+    if args.bp_config >= 0:
+        ctx.set_bp_config(args.bp_config)
 
     def step():
         ctx._call("ct_hip_reconstruct_volume_dev", ctx.handle, d_vol.ptr, n, n, n, na, ftype,

@@ -83,20 +83,35 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
     return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
 
+// Global-memory views of the argument pointers: the persistent kernel reads
+// its arguments from a work list in memory, where the compiler only sees
+// generic pointers; naming the address space keeps its loads and stores
+// global (not flat), as in the per-launch kernel.
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const float __attribute__((address_space(1))) g_cfloat;
+typedef const f2v __attribute__((address_space(1))) g_cfloat2;
+typedef f4v __attribute__((address_space(1))) g_float4;
+
 // One (angle, z-block) work item: 4 rows -> 2 complex FFTs of n points in LDS.
 __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb,
                                                 float2* __restrict__ sm) {
     const int n = p.nfft, logn = p.logn;
     float2* X0 = sm;
     float2* X1 = sm + n;
-    const float2* __restrict__ T = p.tw;  // stage-ordered, L1-resident
+    g_cfloat2* __restrict__ Tg = (g_cfloat2*)p.tw;  // stage-ordered, L1-resident
+    auto T = [Tg](int k) {
+        const f2v t = Tg[k];
+        return make_float2(t.x, t.y);
+    };
+    g_cfloat* __restrict__ H = (g_cfloat*)p.H;
     const int tid = threadIdx.x;
     const int nh = n / 2;
-    const float* rows[kZB];
+    g_cfloat* rows[kZB];
 #pragma unroll
     for (int r = 0; r < kZB; ++r) {
         const int z = zb * kZB + r;
-        rows[r] = (z < p.nz) ? p.sino + ((size_t)z * p.na + a) * p.bins : nullptr;
+        rows[r] = (z < p.nz) ? (g_cfloat*)p.sino + ((size_t)z * p.na + a) * p.bins : nullptr;
     }
     for (int j = tid; j < n; j += 256) {
         const int jj = j - p.pad;
@@ -132,7 +147,7 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
         for (int j = tid; j < nq; j += 256) {
             const int g = j / h, k = j - g * h;
             const int e0 = g * 4 * h + k, e1 = e0 + h, e2 = e0 + 2 * h, e3 = e0 + 3 * h;
-            const float2 w1 = T[h - 1 + k], w2 = T[2 * h - 1 + k], w3 = T[3 * h - 1 + k];
+            const float2 w1 = T(h - 1 + k), w2 = T(2 * h - 1 + k), w3 = T(3 * h - 1 + k);
 #pragma unroll
             for (int f = 0; f < 2; ++f) {
                 float2* X = f ? X1 : X0;
@@ -151,7 +166,7 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     }
     // multiply by the real, even filter (H[n-k] = H[k])
     for (int k = tid; k < n; k += 256) {
-        const float hv = p.H[k <= nh ? k : n - k];
+        const float hv = H[k <= nh ? k : n - k];
         X0[k] = make_float2(X0[k].x * hv, X0[k].y * hv);
         X1[k] = make_float2(X1[k].x * hv, X1[k].y * hv);
     }
@@ -163,7 +178,7 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
         for (int j = tid; j < nq; j += 256) {
             const int g = j / h, k = j - g * h;
             const int e0 = g * 4 * h + k, e1 = e0 + h, e2 = e0 + 2 * h, e3 = e0 + 3 * h;
-            const float2 w2 = T[2 * h - 1 + k], w3 = T[3 * h - 1 + k], w1 = T[h - 1 + k];
+            const float2 w2 = T(2 * h - 1 + k), w3 = T(3 * h - 1 + k), w1 = T(h - 1 + k);
 #pragma unroll
             for (int f = 0; f < 2; ++f) {
                 float2* X = f ? X1 : X0;
@@ -194,11 +209,12 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
         }
     }
     const float inv_n = 1.0f / (float)n;
-    float4* dst = p.filt + ((size_t)zb * p.na + a) * p.sq;
+    g_float4* dst = (g_float4*)p.filt + ((size_t)zb * p.na + a) * p.sq;
     for (int i = tid; i < p.sq; i += 256) {
         const int br = (int)(__brev((unsigned)i) >> (32 - logn));
         const float2 u = X0[br], v = X1[br];
-        dst[i] = make_float4(u.x * inv_n, u.y * inv_n, v.x * inv_n, v.y * inv_n);
+        const f4v o = {u.x * inv_n, u.y * inv_n, v.x * inv_n, v.y * inv_n};
+        dst[i] = o;
     }
     __syncthreads();  // LDS is reused by the next item
 }
