@@ -97,7 +97,9 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
         for (int aa = 0; aa < nc; ++aa) {
             const float2 cs = sm.cs[aa];
             const float c = cs.x, s = cs.y;
-            const int lo = sm.lo[aa];
+            // byte offset of this angle's window entry for bin 0: entry
+            // i0 - lo of row aa sits at i0*16 + boff
+            const int boff = __builtin_amdgcn_readfirstlane((aa * F - sm.lo[aa]) * 16);
             float xc[V], ys[V];
 #pragma unroll
             for (int i = 0; i < V; ++i) {
@@ -110,12 +112,27 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
                 for (int i = 0; i < V; ++i) {
                     const float t = xc[i] - ys[j];
                     const float u = t + dcb;
-                    const float fl = floorf(u);
-                    const int i0 = (int)fl;
-                    const float fr = u - fl;
-                    const int jj = min(max(i0 - lo, 0), F - 1);
-                    const float4 f = sm.f[aa][jj];
-                    const float4 d = sm.d[aa][jj];
+                    int i0;
+                    float fr;
+                    if (CHECK) {
+                        const float fl = floorf(u);
+                        i0 = (int)fl;
+                        fr = u - fl;
+                    } else {
+                        // u >= dcb - R >= 1 inside the circle: truncation is
+                        // floor and fract is u - floor(u), exactly; voxels
+                        // outside the circle are zeroed below
+                        i0 = (int)u;
+                        fr = __builtin_amdgcn_fractf(u);
+                    }
+                    // i0 - lo is in [1, F - 2]: u is monotone in x and y, so
+                    // floor(u) lies between the tile corners' floors, at most
+                    // ceil((T-1)*sqrt2) apart (truncation adds at most 1)
+                    const int bo = i0 * 16 + boff;
+                    const float4 f = *reinterpret_cast<const float4*>(
+                        reinterpret_cast<const char*>(&sm.f[0][0]) + bo);
+                    const float4 d = *reinterpret_cast<const float4*>(
+                        reinterpret_cast<const char*>(&sm.d[0][0]) + bo);
                     if (!CHECK || (i0 >= 0 && i0 + 1 < p.sq)) {
                         acc[i][j].x += fmaf(fr, d.x, f.x);
                         acc[i][j].y += fmaf(fr, d.y, f.y);
