@@ -16,11 +16,13 @@
 // Filtering: the filter is real and even, so two real rows ride in one
 // complex FFT (row A in the real part, row B in the imaginary part) and come
 // back separated. A block owns one angle of one z-block (4 rows -> 2 complex
-// FFTs) held in LDS: decimation-in-time forward from bit-reversed input,
-// multiply by H in natural order, decimation-in-frequency inverse whose
-// output is read back at bit-reversed positions. Twiddles are read from a
-// stage-ordered table in global memory (small enough to stay in L1), at
-// consecutive addresses across a wave.
+// FFTs) held in LDS: decimation-in-frequency forward from natural-order input
+// (bit-reversed spectrum), multiply by H at bit-reversed positions,
+// decimation-in-time inverse back to natural order, so rows are loaded and
+// stored contiguously and no element is moved to its bit-reversed place.
+// Radix-2 stages run fused three at a time in registers. Twiddles are read
+// from a stage-ordered table in global memory (small enough to stay in L1),
+// at consecutive addresses across a wave.
 // Output is written z-innermost (float4 = 4 slices per bin), the layout the
 // backprojection stages from.
 #include "ct_internal.h"
@@ -93,6 +95,192 @@ typedef const float __attribute__((address_space(1))) g_cfloat;
 typedef const f2v __attribute__((address_space(1))) g_cfloat2;
 typedef f4v __attribute__((address_space(1))) g_float4;
 
+// Radix-2 butterfly stages on the two LDS transforms X0, X1 of n points.
+// Stage of half-span m, element e at position q = e mod 2m < m paired with
+// e + m; W = W_2m^q (stage order: W_M^j at M/2 - 1 + j), conjugated when
+// CONJ (the inverse transform):
+//   DIT (increasing spans, bit-reversed in, natural out):
+//       t = W*b;  a' = a + t;  b' = a - t
+//   DIF (decreasing spans, natural in, bit-reversed out):
+//       a' = a + b;  b' = W*(a - b)
+// A fused pass applies 1, 2 or 3 consecutive stages to groups of 2, 4 or 8
+// elements in registers, then writes them back; every stage's operations
+// and their order are those of the single stage.
+template <bool CONJ>
+__device__ __forceinline__ float2 twmul(float2 a, float2 w) {
+    return CONJ ? cmulc(a, w) : cmul(a, w);
+}
+
+template <bool CONJ, class TW>
+__device__ __forceinline__ void fft_dit8(float2* X0, float2* X1, int n, int h, const TW& T,
+                                         int tid) {
+    for (int j = tid; j < n / 8; j += 256) {
+        const int g = j / h, k = j - g * h;
+        const int base = g * 8 * h + k;
+        const float2 w1 = T(h - 1 + k);
+        const float2 w2a = T(2 * h - 1 + k), w2b = T(3 * h - 1 + k);
+        float2 w4[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w4[r] = T(4 * h - 1 + k + r * h);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            float2* X = f ? X1 : X0;
+            float2 x[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) x[r] = X[base + r * h];
+#pragma unroll
+            for (int r = 0; r < 8; r += 2) {  // half-span h
+                const float2 t = twmul<CONJ>(x[r + 1], w1);
+                x[r + 1] = make_float2(x[r].x - t.x, x[r].y - t.y);
+                x[r] = make_float2(x[r].x + t.x, x[r].y + t.y);
+            }
+#pragma unroll
+            for (int r = 0; r < 8; r += 4) {  // half-span 2h
+                const float2 t0 = twmul<CONJ>(x[r + 2], w2a);
+                x[r + 2] = make_float2(x[r].x - t0.x, x[r].y - t0.y);
+                x[r] = make_float2(x[r].x + t0.x, x[r].y + t0.y);
+                const float2 t1 = twmul<CONJ>(x[r + 3], w2b);
+                x[r + 3] = make_float2(x[r + 1].x - t1.x, x[r + 1].y - t1.y);
+                x[r + 1] = make_float2(x[r + 1].x + t1.x, x[r + 1].y + t1.y);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {  // half-span 4h
+                const float2 t = twmul<CONJ>(x[r + 4], w4[r]);
+                x[r + 4] = make_float2(x[r].x - t.x, x[r].y - t.y);
+                x[r] = make_float2(x[r].x + t.x, x[r].y + t.y);
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) X[base + r * h] = x[r];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool CONJ, class TW>
+__device__ __forceinline__ void fft_dif8(float2* X0, float2* X1, int n, int h, const TW& T,
+                                         int tid) {
+    for (int j = tid; j < n / 8; j += 256) {
+        const int g = j / h, k = j - g * h;
+        const int base = g * 8 * h + k;
+        const float2 w1 = T(h - 1 + k);
+        const float2 w2a = T(2 * h - 1 + k), w2b = T(3 * h - 1 + k);
+        float2 w4[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w4[r] = T(4 * h - 1 + k + r * h);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            float2* X = f ? X1 : X0;
+            float2 x[8];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) x[r] = X[base + r * h];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {  // half-span 4h
+                const float2 d = make_float2(x[r].x - x[r + 4].x, x[r].y - x[r + 4].y);
+                x[r] = make_float2(x[r].x + x[r + 4].x, x[r].y + x[r + 4].y);
+                x[r + 4] = twmul<CONJ>(d, w4[r]);
+            }
+#pragma unroll
+            for (int r = 0; r < 8; r += 4) {  // half-span 2h
+                const float2 d0 = make_float2(x[r].x - x[r + 2].x, x[r].y - x[r + 2].y);
+                x[r] = make_float2(x[r].x + x[r + 2].x, x[r].y + x[r + 2].y);
+                x[r + 2] = twmul<CONJ>(d0, w2a);
+                const float2 d1 = make_float2(x[r + 1].x - x[r + 3].x, x[r + 1].y - x[r + 3].y);
+                x[r + 1] = make_float2(x[r + 1].x + x[r + 3].x, x[r + 1].y + x[r + 3].y);
+                x[r + 3] = twmul<CONJ>(d1, w2b);
+            }
+#pragma unroll
+            for (int r = 0; r < 8; r += 2) {  // half-span h
+                const float2 d = make_float2(x[r].x - x[r + 1].x, x[r].y - x[r + 1].y);
+                x[r] = make_float2(x[r].x + x[r + 1].x, x[r].y + x[r + 1].y);
+                x[r + 1] = twmul<CONJ>(d, w1);
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) X[base + r * h] = x[r];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool CONJ, class TW>
+__device__ __forceinline__ void fft_dit4(float2* X0, float2* X1, int n, int h, const TW& T,
+                                         int tid) {
+    for (int j = tid; j < n / 4; j += 256) {
+        const int g = j / h, k = j - g * h;
+        const int base = g * 4 * h + k;
+        const float2 w1 = T(h - 1 + k), w2a = T(2 * h - 1 + k), w2b = T(3 * h - 1 + k);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            float2* X = f ? X1 : X0;
+            float2 x[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) x[r] = X[base + r * h];
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {
+                const float2 t = twmul<CONJ>(x[r + 1], w1);
+                x[r + 1] = make_float2(x[r].x - t.x, x[r].y - t.y);
+                x[r] = make_float2(x[r].x + t.x, x[r].y + t.y);
+            }
+            const float2 t0 = twmul<CONJ>(x[2], w2a);
+            x[2] = make_float2(x[0].x - t0.x, x[0].y - t0.y);
+            x[0] = make_float2(x[0].x + t0.x, x[0].y + t0.y);
+            const float2 t1 = twmul<CONJ>(x[3], w2b);
+            x[3] = make_float2(x[1].x - t1.x, x[1].y - t1.y);
+            x[1] = make_float2(x[1].x + t1.x, x[1].y + t1.y);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) X[base + r * h] = x[r];
+        }
+    }
+    __syncthreads();
+}
+
+template <bool CONJ, class TW>
+__device__ __forceinline__ void fft_dif4(float2* X0, float2* X1, int n, int h, const TW& T,
+                                         int tid) {
+    for (int j = tid; j < n / 4; j += 256) {
+        const int g = j / h, k = j - g * h;
+        const int base = g * 4 * h + k;
+        const float2 w1 = T(h - 1 + k), w2a = T(2 * h - 1 + k), w2b = T(3 * h - 1 + k);
+#pragma unroll
+        for (int f = 0; f < 2; ++f) {
+            float2* X = f ? X1 : X0;
+            float2 x[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) x[r] = X[base + r * h];
+            const float2 d0 = make_float2(x[0].x - x[2].x, x[0].y - x[2].y);
+            x[0] = make_float2(x[0].x + x[2].x, x[0].y + x[2].y);
+            x[2] = twmul<CONJ>(d0, w2a);
+            const float2 d1 = make_float2(x[1].x - x[3].x, x[1].y - x[3].y);
+            x[1] = make_float2(x[1].x + x[3].x, x[1].y + x[3].y);
+            x[3] = twmul<CONJ>(d1, w2b);
+#pragma unroll
+            for (int r = 0; r < 4; r += 2) {
+                const float2 d = make_float2(x[r].x - x[r + 1].x, x[r].y - x[r + 1].y);
+                x[r] = make_float2(x[r].x + x[r + 1].x, x[r].y + x[r + 1].y);
+                x[r + 1] = twmul<CONJ>(d, w1);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) X[base + r * h] = x[r];
+        }
+    }
+    __syncthreads();
+}
+
+// half-span 1 (twiddle 1)
+__device__ __forceinline__ void fft_dit2(float2* X0, float2* X1, int n, int tid) {
+    for (int b = tid; b < n / 2; b += 256) {
+        const int i0 = 2 * b, i1 = i0 + 1;
+        const float2 u0 = X0[i0], v0 = X0[i1], u1 = X1[i0], v1 = X1[i1];
+        X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
+        X0[i1] = make_float2(u0.x - v0.x, u0.y - v0.y);
+        X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
+        X1[i1] = make_float2(u1.x - v1.x, u1.y - v1.y);
+    }
+    __syncthreads();
+}
+__device__ __forceinline__ void fft_dif2(float2* X0, float2* X1, int n, int tid) {
+    fft_dit2(X0, X1, n, tid);  // with twiddle 1 both forms are a + b, a - b
+}
+
 // One (angle, z-block) work item: 4 rows -> 2 complex FFTs of n points in LDS.
 __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb,
                                                 float2* __restrict__ sm) {
@@ -119,100 +307,46 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
 #pragma unroll
         for (int r = 0; r < kZB; ++r)
             v[r] = (rows[r] && jj >= 0 && jj < p.bins) ? rows[r][jj] : 0.0f;
-        const int br = (int)(__brev((unsigned)j) >> (32 - logn));
-        X0[br] = make_float2(v[0], v[1]);
-        X1[br] = make_float2(v[2], v[3]);
+        X0[j] = make_float2(v[0], v[1]);
+        X1[j] = make_float2(v[2], v[3]);
     }
     __syncthreads();
-    // Forward: decimation in time from bit-reversed input, natural-order
-    // output. Radix-2 stages of half-span h are fused in pairs (h, 2h): a
-    // thread takes the quadruple base + {0, h, 2h, 3h} through both stages in
-    // registers, the same operations as two radix-2 stages with half the LDS
-    // round trips. An odd number of stages starts with one radix-2 stage.
-    int h = 1;
-    if (logn & 1) {
-        for (int b = tid; b < nh; b += 256) {
-            const int i0 = 2 * b, i1 = i0 + 1;
-            const float2 u0 = X0[i0], v0 = X0[i1], u1 = X1[i0], v1 = X1[i1];
-            X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
-            X0[i1] = make_float2(u0.x - v0.x, u0.y - v0.y);
-            X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
-            X1[i1] = make_float2(u1.x - v1.x, u1.y - v1.y);
-        }
-        __syncthreads();
-        h = 2;
-    }
-    const int nq = n / 4;
-    for (; h < n; h <<= 2) {
-        for (int j = tid; j < nq; j += 256) {
-            const int g = j / h, k = j - g * h;
-            const int e0 = g * 4 * h + k, e1 = e0 + h, e2 = e0 + 2 * h, e3 = e0 + 3 * h;
-            const float2 w1 = T(h - 1 + k), w2 = T(2 * h - 1 + k), w3 = T(3 * h - 1 + k);
-#pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                float2* X = f ? X1 : X0;
-                const float2 x0 = X[e0], x1 = cmul(X[e1], w1), x2 = X[e2], x3 = cmul(X[e3], w1);
-                const float2 a0 = make_float2(x0.x + x1.x, x0.y + x1.y);
-                const float2 a1 = make_float2(x0.x - x1.x, x0.y - x1.y);
-                const float2 a2 = cmul(make_float2(x2.x + x3.x, x2.y + x3.y), w2);
-                const float2 a3 = cmul(make_float2(x2.x - x3.x, x2.y - x3.y), w3);
-                X[e0] = make_float2(a0.x + a2.x, a0.y + a2.y);
-                X[e2] = make_float2(a0.x - a2.x, a0.y - a2.y);
-                X[e1] = make_float2(a1.x + a3.x, a1.y + a3.y);
-                X[e3] = make_float2(a1.x - a3.x, a1.y - a3.y);
-            }
-        }
-        __syncthreads();
-    }
-    // multiply by the real, even filter (H[n-k] = H[k])
-    for (int k = tid; k < n; k += 256) {
+    // Forward: decimation in frequency from natural-order input to
+    // bit-reversed output; the filter is applied in that order; the inverse
+    // is a decimation in time from bit-reversed input to natural-order output.
+    // No element is ever moved to its bit-reversed place. Radix-2 stages are
+    // fused three at a time (a thread takes the 8 elements base + r*h through
+    // the stages of half-span 4h, 2h, h, or h, 2h, 4h, in registers: the same
+    // operations as the separate stages with one LDS round trip); the 1 or 2
+    // remaining stages run as one radix-2 or radix-4 pass.
+    const int rem = logn % 3;
+    int h = n / 8;
+    for (int pass = 0; pass < logn / 3; ++pass, h >>= 3)
+        fft_dif8<false>(X0, X1, n, h, T, tid);
+    if (rem == 1) fft_dif2(X0, X1, n, tid);
+    else if (rem == 2) fft_dif4<false>(X0, X1, n, 1, T, tid);
+    // multiply by the real, even filter (H[n-k] = H[k]); element p holds
+    // frequency brev(p)
+    for (int q = tid; q < n; q += 256) {
+        const int k = (int)(__brev((unsigned)q) >> (32 - logn));
         const float hv = H[k <= nh ? k : n - k];
-        X0[k] = make_float2(X0[k].x * hv, X0[k].y * hv);
-        X1[k] = make_float2(X1[k].x * hv, X1[k].y * hv);
+        X0[q] = make_float2(X0[q].x * hv, X0[q].y * hv);
+        X1[q] = make_float2(X1[q].x * hv, X1[q].y * hv);
     }
     __syncthreads();
-    // Inverse: decimation in frequency from natural-order input, bit-reversed
-    // output; stages of half-span 2h and h fused in pairs, then (odd count)
-    // one final radix-2 stage of half-span 1.
-    for (h = n / 4; h >= 1; h >>= 2) {
-        for (int j = tid; j < nq; j += 256) {
-            const int g = j / h, k = j - g * h;
-            const int e0 = g * 4 * h + k, e1 = e0 + h, e2 = e0 + 2 * h, e3 = e0 + 3 * h;
-            const float2 w2 = T(2 * h - 1 + k), w3 = T(3 * h - 1 + k), w1 = T(h - 1 + k);
-#pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                float2* X = f ? X1 : X0;
-                const float2 x0 = X[e0], x1 = X[e1], x2 = X[e2], x3 = X[e3];
-                const float2 a0 = make_float2(x0.x + x2.x, x0.y + x2.y);
-                const float2 a2 = cmulc(make_float2(x0.x - x2.x, x0.y - x2.y), w2);
-                const float2 a1 = make_float2(x1.x + x3.x, x1.y + x3.y);
-                const float2 a3 = cmulc(make_float2(x1.x - x3.x, x1.y - x3.y), w3);
-                X[e0] = make_float2(a0.x + a1.x, a0.y + a1.y);
-                X[e1] = cmulc(make_float2(a0.x - a1.x, a0.y - a1.y), w1);
-                X[e2] = make_float2(a2.x + a3.x, a2.y + a3.y);
-                X[e3] = cmulc(make_float2(a2.x - a3.x, a2.y - a3.y), w1);
-            }
-        }
-        __syncthreads();
-        if (h == 1) break;
-        if (h == 2) {  // odd number of stages: finish with half-span 1
-            for (int b = tid; b < nh; b += 256) {
-                const int i0 = 2 * b, i1 = i0 + 1;
-                const float2 u0 = X0[i0], v0 = X0[i1], u1 = X1[i0], v1 = X1[i1];
-                X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
-                X0[i1] = make_float2(u0.x - v0.x, u0.y - v0.y);
-                X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
-                X1[i1] = make_float2(u1.x - v1.x, u1.y - v1.y);
-            }
-            __syncthreads();
-            break;
-        }
+    h = 1;
+    if (rem == 1) {
+        fft_dit2(X0, X1, n, tid);
+        h = 2;
+    } else if (rem == 2) {
+        fft_dit4<true>(X0, X1, n, 1, T, tid);
+        h = 4;
     }
+    for (; h < n; h <<= 3) fft_dit8<true>(X0, X1, n, h, T, tid);
     const float inv_n = 1.0f / (float)n;
     g_float4* dst = (g_float4*)p.filt + ((size_t)zb * p.na + a) * p.sq;
     for (int i = tid; i < p.sq; i += 256) {
-        const int br = (int)(__brev((unsigned)i) >> (32 - logn));
-        const float2 u = X0[br], v = X1[br];
+        const float2 u = X0[i], v = X1[i];
         const f4v o = {u.x * inv_n, u.y * inv_n, v.x * inv_n, v.y * inv_n};
         dst[i] = o;
     }
