@@ -90,6 +90,22 @@ def test_reconstruct_slice_all_filters(backend, O, filt):
           f"vs FP64 {O.rel_l2(got, ref64):.2e}")
 
 
+@pytest.mark.parametrize("bins", [16, 40, 80, 150, 300, 600, 1000])
+def test_fft_filter_every_length(backend, O, bins):
+    """Detector widths whose padded rows need every transform length from 64
+    to 4096 (each remainder of log2(nfft) mod 3 of the fused passes)."""
+    nfft = O.geometry(bins)[2]
+    assert nfft == {16: 64, 40: 128, 80: 256, 150: 512, 300: 1024, 600: 2048, 1000: 4096}[bins]
+    rng = np.random.default_rng(bins)
+    na = 6
+    sino = rng.standard_normal((na, bins)).astype(np.float32)
+    ang = O.serial_angles(na)
+    for filt in (ct.FilterType.Ramp, ct.FilterType.Hann):
+        got = backend.reconstructSlice(ct.Sinogram(sino, ang), 48, ct.ReconstructionParams(filt, na))
+        ref = O.reconstruct_slice(sino, ang, 48, int(filt))
+        assert O.rel_l2(got, ref) <= TOL, (bins, filt)
+
+
 def test_filter_design_on_device(backend, O):
     for n in (64, 512, 2048, 4096):
         for t in range(6):
