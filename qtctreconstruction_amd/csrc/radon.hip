@@ -369,8 +369,9 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
     int y0, y1;
     RayGeom g = ray_geom(live ? i : bins - 1, bins, w, h, c, sn, y0, y1);
     if (!live) g.i = g.im1 = kNoBin;  // walks along, adds nothing
-    const size_t wp = (size_t)(w + kPadX);
-    const float4* __restrict__ slab = rowL + (size_t)zb * h * wp * 2;
+    const int wp = w + kPadX;
+    const char* __restrict__ slab =
+        reinterpret_cast<const char*>(rowL) + (size_t)zb * radon_zbytes(w, h);
     // the extreme bin's geometry, held by the first RR lanes only: lane r
     // computes the window start of row ys + r of the next step
     const int ie = c > 0.0f ? b0 : min(b0 + 255, bins - 1);
@@ -394,21 +395,16 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
             float sd;
             row_setup<false>(ge, ys + RR + threadIdx.x, xw_s[buf ^ 1][threadIdx.x], xd, sd);
         }
-#pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            const int y = ys + r;
-            const int xw = xw_s[buf][r];
-            for (int j = threadIdx.x; j < kRowW; j += 256) {
-                const int x = xw + j;
-                float4 v0 = make_float4(0.f, 0.f, 0.f, 0.f), v1 = v0;
-                if (y < h && x < w + kPadX) {  // pixels past the pad are zero too
-                    const float4* q = slab + ((size_t)y * wp + x) * 2;
-                    v0 = q[0];
-                    v1 = q[1];
-                }
-                t0[r][j] = v0;
-                t1[r][j] = v1;
-            }
+        // Lanes read window slots up to x0 + 2 <= w + 2, inside the zero pad,
+        // and rows below h only: slots past the pad or the last row are never
+        // read, so their addresses are clamped instead of tested.
+        for (int e = threadIdx.x; e < RR * kRowW; e += 256) {
+            const int r = e / kRowW, j = e - r * kRowW;
+            const int y = min(ys + r, h - 1);
+            const int x = min(xw_s[buf][r] + j, w + kPadX - 1);
+            const char* q = slab + (((unsigned)y * (unsigned)wp + (unsigned)x) << 5);
+            (&t0[0][0])[e] = *reinterpret_cast<const float4*>(q);
+            (&t1[0][0])[e] = *reinterpret_cast<const float4*>(q + 16);
         }
         __syncthreads();
         for (int r = 0; r < RR && ys + r < h; ++r) {
