@@ -73,12 +73,13 @@ def cpu_baseline(vol_host: np.ndarray, na: int, ftype: int, target_s: float):
 
 
 def row_walk_angles(na: int) -> int:
-    """Angles of the Serial set with |sin| <= |cos| (float32 trig, as the host tables)."""
+    """Angles of the Serial set the Radon row walk takes: |sin| <= |cos| or
+    |cos| >= 0.6 (float32 trig, as the host tables)."""
     a = np.arange(na, dtype=np.float32) * np.float32(180.0 / na)
     th = (a * np.float32(3.14159265359)) / np.float32(180.0)
     c = np.cos(th.astype(np.float64)).astype(np.float32)
     s = np.sin(th.astype(np.float64)).astype(np.float32)
-    return int(np.count_nonzero(~(np.abs(s) > np.abs(c))))
+    return int(np.count_nonzero(~(np.abs(s) > np.abs(c)) | (np.abs(c) >= np.float32(0.6))))
 
 
 def dominant_roofline(n, na, stage, radon_k, dir_launches):
@@ -94,9 +95,9 @@ def dominant_roofline(n, na, stage, radon_k, dir_launches):
     cands = {
         "radon_ray_kernel": (radon_k[1], max(dir_launches, 1),
                              FLOP_PER_SAMPLE * samples * (na - n_row),
-                             "Radon direct walk (angles with |sin| > |cos|)"),
+                             "Radon direct walk (angles with |cos| < 0.6)"),
         "radon_rows_kernel": (radon_k[0], 1, FLOP_PER_SAMPLE * samples * n_row,
-                              "Radon LDS row walk (angles with |sin| <= |cos|)"),
+                              "Radon LDS row walk (angles with |cos| >= 0.6)"),
         "bp_kernel": (stage[2], 1, FLOP_PER_UPDATE * samples * na,
                       "voxel-driven backprojection"),
     }

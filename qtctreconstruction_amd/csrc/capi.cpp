@@ -77,20 +77,27 @@ ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
         cs[a] = std::cos(th);
         sn[a] = std::sin(th);
     }
-    // the Radon stage walks rays of angles with |sin| <= |cos| along rows and
-    // the others along columns (the same float test as the kernels')
+    // The Radon stage's angle groups, by the same float tests as the kernels':
+    // |sin| <= |cos| (row walk, 3 candidates), 0.6 <= |cos| < |sin| (row walk,
+    // 4 candidates), the rest (direct walk).
     std::vector<int> alist;
     alist.reserve(na);
+    auto steep = [&](int a) { return std::fabs(sn[a]) > std::fabs(cs[a]); };
+    auto rowcos = [&](int a) { return std::fabs(cs[a]) >= 0.6f; };
     for (int a = 0; a < na; ++a)
-        if (!(std::fabs(sn[a]) > std::fabs(cs[a]))) alist.push_back(a);
-    const int n_row = (int)alist.size();
+        if (!steep(a)) alist.push_back(a);
+    const int n_row3 = (int)alist.size();
     for (int a = 0; a < na; ++a)
-        if (std::fabs(sn[a]) > std::fabs(cs[a])) alist.push_back(a);
+        if (steep(a) && rowcos(a)) alist.push_back(a);
+    const int n_row4 = (int)alist.size() - n_row3;
+    for (int a = 0; a < na; ++a)
+        if (steep(a) && !rowcos(a)) alist.push_back(a);
     ct_hip_ctx::Trig t;
     t.cos = static_cast<float*>(dmalloc(c, sizeof(float) * na));
     t.sin = static_cast<float*>(dmalloc(c, sizeof(float) * na));
     t.alist = static_cast<int*>(dmalloc(c, sizeof(int) * na));
-    t.n_row = n_row;
+    t.n_row3 = n_row3;
+    t.n_row4 = n_row4;
     hip_check(hipMemcpyAsync(t.cos, cs.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
               "upload cos table");
     hip_check(hipMemcpyAsync(t.sin, sn.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
@@ -171,13 +178,14 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     // anything else on the context stream.
     hip_check(hipEventRecord(c->evr[0], c->stream), "event");
     hip_check(hipStreamWaitEvent(c->stream2, c->evr[0], 0), "stream wait");
+    const int n_rows = c->cur.n_row3 + c->cur.n_row4;
     hip_check(launch_radon_rows(row, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
-                                c->cur.n_row, d_sino, c->stream2),
+                                c->cur.n_row3, c->cur.n_row4, d_sino, c->stream2),
               "radon rows");
     hip_check(hipEventRecord(c->evr[1], c->stream2), "event");
     hip_check(launch_radon_direct(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin,
-                                  c->cur.alist + c->cur.n_row, na - c->cur.n_row, d_sino,
-                                  c->stream, &c->radon_direct_launches),
+                                  c->cur.alist + n_rows, na - n_rows, d_sino, c->stream,
+                                  &c->radon_direct_launches),
               "radon direct");
     hip_check(hipEventRecord(c->evr[2], c->stream), "event");
     hip_check(hipStreamWaitEvent(c->stream, c->evr[1], 0), "stream join");

@@ -58,10 +58,11 @@ hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hip
 // pixel (2 float4), kPadX zero pixels past each row / column.
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, hipStream_t s);
-// Radon of the n_row angles alist[0..n_row) (|sin| <= |cos|): LDS-staged row walk
+// Radon of the angles alist[0..n_row3) (|sin| <= |cos|) and
+// alist[n_row3..n_row3+n_row4) (0.6 <= |cos| < |sin|): LDS-staged row walk
 hipError_t launch_radon_rows(const float4* d_row, int w, int h, int nz, int na, int bins,
                              const float* d_cos, const float* d_sin, const int* d_alist,
-                             int n_row, float* d_sino, hipStream_t s);
+                             int n_row3, int n_row4, float* d_sino, hipStream_t s);
 // Radon of the n_dir angles alist[0..n_dir) (|sin| > |cos|): direct walk over
 // both copies, a few z-blocks per launch; *launches gets the launch count
 hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, int h, int nz,

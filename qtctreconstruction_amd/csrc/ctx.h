@@ -74,8 +74,11 @@ struct ct_hip_ctx {
     struct Trig {
         float* cos = nullptr;
         float* sin = nullptr;
-        int* alist = nullptr;  // angle indices: row-walk angles (|sin| <= |cos|) first
-        int n_row = 0;         // how many of them walk rows
+        // angle indices for the Radon stage: n_row3 angles with |sin| <= |cos|,
+        // then n_row4 with 0.6 <= |cos| < |sin| (both walk rows), then the
+        // direct-walk angles
+        int* alist = nullptr;
+        int n_row3 = 0, n_row4 = 0;
     };
     std::map<std::vector<uint32_t>, Trig> trig;  // keyed by the angle list's bit pattern
     Trig cur;                                     // table of the current plain call

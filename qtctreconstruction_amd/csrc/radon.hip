@@ -154,6 +154,10 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
 // [i-1-kDelta, i+1+kDelta] in exact arithmetic contains every pixel with
 // floor(s_float) in {i-1, i}.
 constexpr float kDelta = 1.0f / 32.0f;
+// Angles with |cos| >= kRowCos walk rows (LDS kernel): |sin| <= |cos| with
+// chunks of 3 candidates, 0.6 <= |cos| < |sin| with chunks of 4 (a row then
+// holds at most span + 0.02 <= 2.0625/0.6 + 0.02 < 4 of them).
+constexpr float kRowCos = 0.6f;
 
 // Ray walk. A lane's candidates come in chunks of U consecutive pixels of
 // one row, in Serial's raster order: (row y, x..x+U-1).
@@ -176,7 +180,8 @@ template <bool STEEP>
 __device__ __forceinline__ void row_setup(const RayGeom& g, int y, int& x0, int& x1, float& sy) {
     sy = ((float)y - g.cy) * g.sn;
     if (!STEEP) {
-        // at most 3 candidates per row (span + 0.02 <= 2.0625*sqrt2 + 0.02 < 3)
+        // at most 3 candidates per row when |s| <= |c| (span + 0.02 <=
+        // 2.0625*sqrt2 + 0.02 < 3), at most 4 when |c| >= kRowCos
         const float xa = g.xa0 + (float)y * g.dxa;
         x0 = min(max(0, (int)ceilf(fmaxf(xa - 0.01f, -1.0f))), g.w);
         x1 = x0;
@@ -283,16 +288,17 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
     }
 
     // Per-row candidate interval in x: x - cx = ((s-dc) - (y-cy) sn) / c over
-    // s - dc in [slo, shi]. For |c| >= |s| it is stepped from row to row,
-    // x in [xa0 + y*dxa, xa0 + y*dxa + span] (|dxa| <= 1, |1/c| <= sqrt2, so
-    // the stepping error, ~1e-3 px, is far below the kDelta/|c| margin).
+    // s - dc in [slo, shi]. For |c| >= kRowCos it is stepped from row to row,
+    // x in [xa0 + y*dxa, xa0 + y*dxa + span] (|dxa| <= 4/3, |1/c| <= 5/3, so
+    // the stepping error, ~1e-3 px, is far below the kDelta/|c| and 0.01 px
+    // margins).
     // Steeper rays compute it per row from sy, whose error then scales with
     // the distance to the strip and vanishes where candidates exist.
     g.has_c = ac > 1e-20f;
     g.inv_c = g.has_c ? 1.0f / c : 0.0f;
     g.xa0 = 0.0f;
     g.dxa = 0.0f;
-    if (!(as > ac)) {
+    if (ac >= kRowCos) {
         const float e0 = (g.slo + g.cy * sn) * g.inv_c, e1 = (g.shi + g.cy * sn) * g.inv_c;
         g.xa0 = g.cx + fminf(e0, e1);
         g.dxa = -sn * g.inv_c;
@@ -339,24 +345,27 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
     }
 }
 
-// Rays of angles with |sin| <= |cos|: every lane of a block (256 bins of one
-// angle and z-block) walks all rows top to bottom, at most 3 candidates per
-// row. The block stages RR rows at a time into LDS, each as a window of kRowW
+// Rays of angles with |cos| >= kRowCos: every lane of a block (256 bins of
+// one angle and z-block) walks all rows top to bottom, at most U candidates
+// per row. The block stages RR rows at a time into LDS, each as a window of kRowW
 // pixels starting at the leftmost lane's first candidate. A lane's first
 // candidate x0(y) is a composition of monotone float operations of its bin,
 // so the window start is that of the extreme bin (the lowest one for
 // cos > 0, the highest for cos < 0), and every lane's chunk lies within
-// kRowW pixels of it: lanes 255 bins apart are at most 255/|cos| <= 360.7
-// pixels apart, plus 3 pixels of chunk. Rows outside a lane's candidate rows
-// hold no candidates of its bin and add +0.
-constexpr int kRowW = 368;
+// kRowW pixels of it: lanes 255 bins apart are at most 255/|cos| pixels
+// apart (360.7 for |sin| <= |cos|, 425 for |cos| >= 0.6), plus a chunk of
+// 3 or 4 pixels. Rows outside a lane's candidate rows hold no candidates of
+// its bin and add +0.
 constexpr int kRadonRows = 2;  // rows staged per step
 
-template <int RR>
+template <int RR, int U, int kRowW>
 __global__ __launch_bounds__(256) void radon_rows_kernel(
     const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
     const float* __restrict__ cos_t, const float* __restrict__ sin_t,
     const int* __restrict__ alist, float* __restrict__ sino) {
+    // window: lanes 255 bins apart are at most 255/|cos| pixels apart, plus a
+    // chunk of U; the window must also keep every chunk inside the zero pad
+    static_assert(U - 2 <= kPadX - 1, "chunk reaches past the zero pad");
     __shared__ float4 t0[RR][kRowW];  // slices 0-3 of the staged pixels
     __shared__ float4 t1[RR][kRowW];  // slices 4-7
     __shared__ int xw_s[2][RR];       // window starts: staged rows, next rows
@@ -413,13 +422,13 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
             float sy;
             row_setup<false>(g, y, x0, x1, sy);
             const int d = x0 - xw_s[buf][r];
-            float4 v[3][2];
+            float4 v[U][2];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
+            for (int k = 0; k < U; ++k) {
                 v[k][0] = t0[r][d + k];
                 v[k][1] = t1[r][d + k];
             }
-            chunk_add<3>(g, x0, sy, v, acc);
+            chunk_add<U>(g, x0, sy, v, acc);
         }
     }
     if (live) {
@@ -462,13 +471,24 @@ hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const floa
 
 hipError_t launch_radon_rows(const float4* d_row, int w, int h, int nz, int na, int bins,
                              const float* d_cos, const float* d_sin, const int* d_alist,
-                             int n_row, float* d_sino, hipStream_t s) {
-    if (n_row <= 0) return hipSuccess;
+                             int n_row3, int n_row4, float* d_sino, hipStream_t s) {
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int bb = (bins + 255) / 256;
-    hipLaunchKernelGGL(radon_rows_kernel<kRadonRows>, dim3(bb, n_row, nzb), dim3(256), 0, s,
-                       d_row, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino);
-    return hipGetLastError();
+    if (n_row3 > 0) {  // |sin| <= |cos|: 3 candidates, window 368
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 368>), dim3(bb, n_row3, nzb),
+                           dim3(256), 0, s, d_row, w, h, nz, na, bins, d_cos, d_sin, d_alist,
+                           d_sino);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (n_row4 > 0) {  // 0.6 <= |cos| < |sin|: 4 candidates, window 432
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 4, 432>), dim3(bb, n_row4, nzb),
+                           dim3(256), 0, s, d_row, w, h, nz, na, bins, d_cos, d_sin,
+                           d_alist + n_row3, d_sino);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, int h, int nz,
