@@ -254,10 +254,8 @@ int ct_hip_noisy_log_dev(ct_hip_ctx* ctx, float* d_sino, size_t n, float I0, flo
 int ct_hip_last_timings(const ct_hip_ctx* ctx, double* ms4);
 
 /* Breakdown of the Radon stage of the last volume call that ran one,
- * milliseconds from the end of normalisation, HIP events: [0] the row walk
- * (angles with |sin| <= |cos|, one launch, on the context's second stream),
- * [1] the direct walk (the other angles, on the context's stream); the two
- * run concurrently;
+ * milliseconds, HIP events on the context's stream: [0] the row walk (angles
+ * with |cos| >= 0.6, two launches), [1] the direct walk (the other angles);
  * *direct_launches (optional) gets the direct walk's launch count.
  * No counterpart in the reference: per-kernel timing for profiling and the
  * roofline report. CT_ERR_INVALID before any Radon stage has run. */

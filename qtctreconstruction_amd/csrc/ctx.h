@@ -84,11 +84,8 @@ struct ct_hip_ctx {
     Trig cur;                                     // table of the current plain call
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double timings[4] = {0, 0, 0, 0};
-    // The Radon stage runs its row walk on stream2, beside the direct walk on
-    // stream: evr[0] (stream, after normalisation) starts both, evr[1]
-    // (stream2) ends the row walk and is joined by stream, evr[2] (stream)
-    // ends the direct walk.
-    hipStream_t stream2 = nullptr;
+    // Radon stage breakdown: events before the row walk, between the two
+    // walks and after the direct walk
     hipEvent_t evr[3] = {nullptr, nullptr, nullptr};
     double radon_ms[2] = {0, 0};
     int radon_direct_launches = 0;
