@@ -38,9 +38,9 @@ FLOP_PER_SAMPLE = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_ray_kernel": 1.311e9,    # 1.2994e9 read + 1.18e7 written per launch (2 z-blocks)
-    "radon_rows_kernel": 2.146e10,  # 2.1084e10 read + 3.785e8 written
-    "bp_kernel": 6.528e9,           # 5.9913e9 read + 5.369e8 written
+    "radon_ray_kernel": 1.422e+09,
+    "radon_rows_kernel": 2.067e+10,
+    "bp_kernel": 6.534e+09,
 }
 
 
