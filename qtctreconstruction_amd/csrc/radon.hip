@@ -172,8 +172,9 @@ struct RayGeom {
     float c, sn, cx, cy, dc, slo, shi, inv_c;
     float xa0, dxa;  // row-stepped interval start (|c| >= |s|)
     bool has_c;
-    int w, i, im1;
-    float keep;      // 0 for the last bin (full weight), else 1
+    int w;
+    // factor of this lane's bin: clamp(min(s*ka + la, s*kb + lb), 0, 1)
+    float ka, la, kb, lb;
 };
 
 template <bool STEEP>
@@ -209,21 +210,50 @@ __device__ __forceinline__ void chunk_load(const char* __restrict__ p, unsigned 
     }
 }
 
+// Factor of bin i for a pixel at s. Serial (SerialBackend.cpp:110-120) adds
+// v*(1-frac) to bin i0 = floor(s) and v*frac to bin i0+1, frac = s - i0; the
+// last bin takes v itself when i0 is the last bin; nothing when i0 < 0. With
+//   a = s - (i-1),  b = (i+1) - s,  fac = clamp(min(a, b), 0, 1):
+// - s in [i-1, i): Serial's weight is frac = fl(s - (i-1)), the same float
+//   operation as a, and a <= 1 <= b;
+// - s in [i, i+1): Serial's is fl(1 - fl(s - i)); s - i is exact (Sterbenz
+//   for i >= 1, trivially for i = 0), so both round the same real number
+//   (i+1) - s, and b <= 1 <= a;
+// - elsewhere a <= 0 or b <= 0 (rounding is monotone), so fac = 0.
+// Bin 0 takes nothing for s < 0: its a = s*2^64 + 1 is >= 1 for s >= 0 and
+// negative for s < 0 (a nonzero s is a multiple of 2^-49: |cos|, |sin| are 0
+// or above 2^-25 and pixel offsets are multiples of 1/2). The last bin takes
+// v for s in [i, i+1): its b = (i+1-s)*2^64 is >= 1 there (s is a float
+// below i+1) and <= 0 from i+1 on, so the clamp gives exactly 1. Each of a
+// and b is one fma: fl(s*1 + c) = fl(s + c). The products v*fac are rounded
+// before they are added, as in Serial; a zero factor adds +0 or -0, which
+// leaves a partial sum >= +0 unchanged.
+__device__ __forceinline__ void bin_factors(RayGeom& g, int i, int bins) {
+    constexpr float kBig = 18446744073709551616.0f;  // 2^64
+    g.ka = 1.0f;
+    g.la = -(float)(i - 1);
+    g.kb = -1.0f;
+    g.lb = (float)(i + 1);
+    if (i == 0) {  // SerialBackend.cpp:113 (i0 >= 0)
+        g.ka = kBig;
+        g.la = 1.0f;
+    }
+    if (i == bins - 1) {  // SerialBackend.cpp:117-118
+        g.kb = -kBig;
+        g.lb = (float)(i + 1) * kBig;
+    }
+}
+
 template <int U>
 __device__ __forceinline__ void chunk_add(const RayGeom& g, int x, float sy, const float4 (&v)[U][2],
                                           float (&acc)[kRZ]) {
     const float xm0 = (float)x - g.cx;
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-        // SerialBackend.cpp:110-120, same float operations and order: s, its
-        // floor and fraction (exact for s >= 0; s < 0 gives i0 <= -1, which
-        // no lane's candidate test accepts), and the two bin factors
+        // SerialBackend.cpp:110, same float operations and order
         const float s = (xm0 + (float)k) * g.c + sy + g.dc;
-        const int i0 = (int)floorf(s);
-        const float frac = __builtin_amdgcn_fractf(s);
-        const float f_here = fmaf(-frac, g.keep, 1.0f);  // 1 - frac, or 1 for the last bin
-        // non-candidates add +0, which leaves every partial sum unchanged
-        const float fac = (i0 == g.i) ? f_here : ((i0 == g.im1) ? frac : 0.0f);
+        const float a = fmaf(s, g.ka, g.la), b = fmaf(s, g.kb, g.lb);
+        const float fac = __builtin_amdgcn_fmed3f(fminf(a, b), 0.0f, 1.0f);
         acc[0] += v[k][0].x * fac;
         acc[1] += v[k][0].y * fac;
         acc[2] += v[k][0].z * fac;
@@ -267,9 +297,7 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
     g.cy = (float)(h - 1) * 0.5f;
     g.dc = (float)(bins / 2);
     g.w = w;
-    g.i = i;
-    g.im1 = (i >= 1) ? i - 1 : -0x7fffffff;
-    g.keep = (i == bins - 1) ? 0.0f : 1.0f;  // SerialBackend.cpp:113-116
+    bin_factors(g, i, bins);
     g.slo = (float)(i - 1) - kDelta - g.dc;  // s - dc lower bound
     g.shi = (float)(i + 1) + kDelta - g.dc;  // s - dc upper bound
     const float ac = fabsf(c), as = fabsf(sn);
@@ -305,9 +333,6 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
     }
     return g;
 }
-
-// A bin no floor(s) equals: a lane holding it adds only +0.
-constexpr int kNoBin = -0x7fffffff - 1;
 
 // Rays of angles with |sin| > |cos|: each lane walks its rows with direct
 // loads. Lanes whose ray enters through the top edge (first candidate row
@@ -377,7 +402,10 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
     const float c = cos_t[a], sn = sin_t[a];
     int y0, y1;
     RayGeom g = ray_geom(live ? i : bins - 1, bins, w, h, c, sn, y0, y1);
-    if (!live) g.i = g.im1 = kNoBin;  // walks along, adds nothing
+    if (!live) {  // walks along, adds nothing: factor clamp(-1) = 0
+        g.ka = g.kb = 0.0f;
+        g.la = g.lb = -1.0f;
+    }
     const int wp = w + kPadX;
     const char* __restrict__ slab =
         reinterpret_cast<const char*>(rowL) + (size_t)zb * radon_zbytes(w, h);

@@ -42,7 +42,7 @@ def _slice(rng, h, w, zero_frac=0.3):
 
 @pytest.mark.parametrize("h,w,na,bins", [(64, 64, 180, 64), (97, 97, 33, 97), (50, 64, 45, 70),
                                          (64, 50, 7, 40), (32, 32, 1, 32), (128, 128, 360, 128),
-                                         (20, 20, 12, 3)])
+                                         (20, 20, 12, 3), (9, 8, 16, 1), (12, 10, 30, 2)])
 def test_sinogram_bit_exact(backend, O, h, w, na, bins):
     rng = np.random.default_rng(h * 1000 + na)
     s = _slice(rng, h, w)
