@@ -170,32 +170,34 @@ constexpr float kRowCos = 0.6f;
 // keeps every position within the kPadX pad.
 struct RayGeom {
     float c, sn, cx, cy, dc, slo, shi, inv_c;
-    float xa0, dxa;  // row-stepped interval start (|c| >= |s|)
+    float xa0, dxa;  // row-stepped interval start, less 0.01 px (|c| >= kRowCos)
     bool has_c;
     int w;
     // factor of this lane's bin: clamp(min(s*ka + la, s*kb + lb), 0, 1)
     float ka, la, kb, lb;
 };
 
-template <bool STEEP>
-__device__ __forceinline__ void row_setup(const RayGeom& g, int y, int& x0, int& x1, float& sy) {
+// First candidate column of row yf for |c| >= kRowCos, as a float in
+// [0, w]: ceil of the stepped interval start clamped to [0, w] (ceil and the
+// clamp commute for an integer w). The row then holds at most 3 candidates
+// when |s| <= |c| (span + 0.02 <= 2.0625*sqrt2 + 0.02 < 3), at most 4 when
+// |c| >= kRowCos. Monotone in the bin: every operation is.
+__device__ __forceinline__ float row_first(const RayGeom& g, float yf) {
+    return ceilf(__builtin_amdgcn_fmed3f(fmaf(yf, g.dxa, g.xa0), 0.0f, (float)g.w));
+}
+
+// Candidate columns x0..x1 of row y for |c| < kRowCos, and Serial's
+// (y - cy)*sin.
+__device__ __forceinline__ void row_span(const RayGeom& g, int y, int& x0, int& x1, float& sy) {
     sy = ((float)y - g.cy) * g.sn;
-    if (!STEEP) {
-        // at most 3 candidates per row when |s| <= |c| (span + 0.02 <=
-        // 2.0625*sqrt2 + 0.02 < 3), at most 4 when |c| >= kRowCos
-        const float xa = g.xa0 + (float)y * g.dxa;
-        x0 = min(max(0, (int)ceilf(fmaxf(xa - 0.01f, -1.0f))), g.w);
-        x1 = x0;
-    } else {
-        x0 = 0;
-        x1 = g.w - 1;
-        if (g.has_c) {
-            const float lo = (g.slo - sy) * g.inv_c;
-            const float hi = (g.shi - sy) * g.inv_c;
-            x0 = max(0, (int)ceilf(fmaxf(g.cx + fminf(lo, hi), -1.0f)));
-            x1 = min(g.w - 1, (int)floorf(fminf(g.cx + fmaxf(lo, hi), (float)g.w)));
-            x0 = min(x0, g.w);
-        }
+    x0 = 0;
+    x1 = g.w - 1;
+    if (g.has_c) {
+        const float lo = (g.slo - sy) * g.inv_c;
+        const float hi = (g.shi - sy) * g.inv_c;
+        x0 = max(0, (int)ceilf(fmaxf(g.cx + fminf(lo, hi), -1.0f)));
+        x1 = min(g.w - 1, (int)floorf(fminf(g.cx + fmaxf(lo, hi), (float)g.w)));
+        x0 = min(x0, g.w);
     }
 }
 
@@ -244,14 +246,16 @@ __device__ __forceinline__ void bin_factors(RayGeom& g, int i, int bins) {
     }
 }
 
+// The candidates x0 + k, k < U, of one row; xm0 = x0 - cx exactly (pixel
+// offsets are multiples of 1/2 far below 2^24).
 template <int U>
-__device__ __forceinline__ void chunk_add(const RayGeom& g, int x, float sy, const float4 (&v)[U][2],
-                                          float (&acc)[kRZ]) {
-    const float xm0 = (float)x - g.cx;
+__device__ __forceinline__ void chunk_add(const RayGeom& g, float xm0, float sy,
+                                          const float4 (&v)[U][2], float (&acc)[kRZ]) {
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-        // SerialBackend.cpp:110, same float operations and order
-        const float s = (xm0 + (float)k) * g.c + sy + g.dc;
+        // SerialBackend.cpp:110, same float operations and order (xm0 is
+        // never -0, so xm0 + 0 is xm0)
+        const float s = (k == 0 ? xm0 : xm0 + (float)k) * g.c + sy + g.dc;
         const float a = fmaf(s, g.ka, g.la), b = fmaf(s, g.kb, g.lb);
         const float fac = __builtin_amdgcn_fmed3f(fminf(a, b), 0.0f, 1.0f);
         acc[0] += v[k][0].x * fac;
@@ -267,7 +271,7 @@ __device__ __forceinline__ void chunk_add(const RayGeom& g, int x, float sy, con
 
 // A lane's walk: rows y0..y1 in order, each as chunks of U pixels from its
 // first candidate. All loads of a chunk are issued before its arithmetic.
-template <bool STEEP, int U>
+template <int U>
 __device__ __forceinline__ void ray_walk(const RayGeom& g, const char* __restrict__ p,
                                          unsigned ystep, unsigned xstep, int y0, int y1,
                                          float (&acc)[kRZ]) {
@@ -277,12 +281,12 @@ __device__ __forceinline__ void ray_walk(const RayGeom& g, const char* __restric
     for (int y = y0; y <= y1; ++y) {
         int x0, x1;
         float sy;
-        row_setup<STEEP>(g, y, x0, x1, sy);
+        row_span(g, y, x0, x1, sy);
         const unsigned roff = (unsigned)y * ystep;
         for (int x = x0; x <= x1; x += U) {
             float4 v[U][2];
             chunk_load<U>(p, roff + (unsigned)x * xstep, xstep, v);
-            chunk_add<U>(g, x, sy, v, acc);
+            chunk_add<U>(g, (float)x - g.cx, sy, v, acc);
         }
     }
 }
@@ -328,7 +332,7 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
     g.dxa = 0.0f;
     if (ac >= kRowCos) {
         const float e0 = (g.slo + g.cy * sn) * g.inv_c, e1 = (g.shi + g.cy * sn) * g.inv_c;
-        g.xa0 = g.cx + fminf(e0, e1);
+        g.xa0 = (g.cx + fminf(e0, e1)) - 0.01f;
         g.dxa = -sn * g.inv_c;
     }
     return g;
@@ -362,7 +366,7 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
     float acc[kRZ];
 #pragma unroll
     for (int k = 0; k < kRZ; ++k) acc[k] = 0.0f;
-    ray_walk<true, U>(g, p, ystep, xstep, y0, y1, acc);
+    ray_walk<U>(g, p, ystep, xstep, y0, y1, acc);
 #pragma unroll
     for (int k = 0; k < kRZ; ++k) {
         const int z = zb * kRZ + k;
@@ -416,10 +420,15 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
     if (threadIdx.x < RR) {
         int ey0, ey1;
         ge = ray_geom(ie, bins, w, h, c, sn, ey0, ey1);
-        int xd;
-        float sd;
-        row_setup<false>(ge, threadIdx.x, xw_s[0][threadIdx.x], xd, sd);
+        xw_s[0][threadIdx.x] = (int)row_first(ge, (float)threadIdx.x);
     }
+    // Staging: waves 0-1 stage the step's first row, waves 2-3 its second;
+    // thread column j = (threadIdx.x & 127) + 128 k, k < NE. Columns past
+    // kRowW load a clamped in-bounds address and are not written.
+    static_assert(RR == 2, "two staged rows, one per wave pair");
+    constexpr int NE = (kRowW + 127) / 128;
+    const int srow = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);
+    const int sj0 = threadIdx.x & 127;
 
     float acc[kRZ];
 #pragma unroll
@@ -427,36 +436,46 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
     int buf = 0;
     for (int ys = 0; ys < h; ys += RR, buf ^= 1) {
         __syncthreads();  // the previous rows are consumed; xw_s[buf] is set
-        if (threadIdx.x < RR) {  // window starts of the next step's rows
-            int xd;
-            float sd;
-            row_setup<false>(ge, ys + RR + threadIdx.x, xw_s[buf ^ 1][threadIdx.x], xd, sd);
-        }
-        // Lanes read window slots up to x0 + 2 <= w + 2, inside the zero pad,
-        // and rows below h only: slots past the pad or the last row are never
-        // read, so their addresses are clamped instead of tested.
-        for (int e = threadIdx.x; e < RR * kRowW; e += 256) {
-            const int r = e / kRowW, j = e - r * kRowW;
-            const int y = min(ys + r, h - 1);
-            const int x = min(xw_s[buf][r] + j, w + kPadX - 1);
-            const char* q = slab + (((unsigned)y * (unsigned)wp + (unsigned)x) << 5);
-            (&t0[0][0])[e] = *reinterpret_cast<const float4*>(q);
-            (&t1[0][0])[e] = *reinterpret_cast<const float4*>(q + 16);
+        if (threadIdx.x < RR)  // window starts of the next step's rows
+            xw_s[buf ^ 1][threadIdx.x] = (int)row_first(ge, (float)(ys + RR + threadIdx.x));
+        // Lanes read window slots up to x0 + U - 1 <= w + kPadX - 1, inside
+        // the zero pad, and rows below h only: slots past the pad or the last
+        // row are never read, so their addresses are clamped instead of
+        // tested. All loads are issued before the LDS writes.
+        {
+            // this wave pair's row (clamped to the last row) and window
+            // start, both wave-uniform
+            const unsigned ro = (unsigned)min(ys + srow, h - 1) * (unsigned)wp * 32u;
+            const int xr = __builtin_amdgcn_readfirstlane(xw_s[buf][srow]);
+            float4 q0[NE], q1[NE];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) {
+                const int x = min(xr + sj0 + 128 * k, w + kPadX - 1);
+                const unsigned off = ro + ((unsigned)x << 5);
+                q0[k] = *reinterpret_cast<const float4*>(slab + off);
+                q1[k] = *reinterpret_cast<const float4*>(slab + off + 16);
+            }
+#pragma unroll
+            for (int k = 0; k < NE; ++k) {
+                if (sj0 + 128 * k < kRowW) {
+                    t0[srow][sj0 + 128 * k] = q0[k];
+                    t1[srow][sj0 + 128 * k] = q1[k];
+                }
+            }
         }
         __syncthreads();
         for (int r = 0; r < RR && ys + r < h; ++r) {
-            const int y = ys + r;
-            int x0, x1;
-            float sy;
-            row_setup<false>(g, y, x0, x1, sy);
-            const int d = x0 - xw_s[buf][r];
+            const float yf = (float)(ys + r);
+            const float sy = (yf - g.cy) * g.sn;  // SerialBackend.cpp:110
+            const float x0f = row_first(g, yf);
+            const int d = (int)x0f - xw_s[buf][r];
             float4 v[U][2];
 #pragma unroll
             for (int k = 0; k < U; ++k) {
                 v[k][0] = t0[r][d + k];
                 v[k][1] = t1[r][d + k];
             }
-            chunk_add<U>(g, x0, sy, v, acc);
+            chunk_add<U>(g, x0f - g.cx, sy, v, acc);
         }
     }
     if (live) {
@@ -539,7 +558,8 @@ hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, 
         if (v >= 1) per = v;
     }
     for (int zb0 = 0; zb0 < nzb; zb0 += per) {
-        // chunks of 4 candidates (8 loads in flight per lane)
+        // chunks of 4 candidates (8 loads in flight per lane); smaller
+        // chunks load fewer pixels past the runs but measured slower
         hipLaunchKernelGGL(radon_ray_kernel<4>, dim3(bb, n_dir, std::min(per, nzb - zb0)),
                            dim3(256), 0, s, d_row, d_col, w, h, nz, na, bins, d_cos, d_sin,
                            d_alist, d_sino, zb0);
