@@ -193,12 +193,14 @@ enum { CT_BATCH_GRAPH = 1,     /* capture the launches once in a hipGraph and re
  * BackendFactory.cpp:17-60). Waits for completion. */
 int ct_hip_fbp_batch_dev(ct_hip_ctx* ctx, const ct_fbp_job* jobs, int njobs, int flags);
 
-/* Backprojection tile configurations (voxel tile edge, angles per LDS stage)
- * and the autotuner that picks one per (N, na) by timing them all on the
- * device (cached in the context and, when CT_HIP_TUNE_CACHE names a file,
- * there). ms_out (optional, n_ms entries) receives each configuration's time. */
+/* Backprojection tile configurations (voxel tile edge, angles per LDS
+ * stage, slices per thread: 4 or 8; slices may be NULL) and the autotuner
+ * that picks one per (N, na) by timing them all on the device (cached in the
+ * context and, when CT_HIP_TUNE_CACHE names a file, there). ms_out
+ * (optional, n_ms entries) receives each configuration's time. Every
+ * configuration gives bit-identical volumes. */
 int ct_hip_bp_config_count(void);
-int ct_hip_bp_config(int index, int* tile, int* angles);
+int ct_hip_bp_config(int index, int* tile, int* angles, int* slices);
 int ct_hip_autotune(ct_hip_ctx* ctx, int N, int na, int* config_out, double* ms_out, int n_ms);
 /* Configuration used by the per-volume calls (default 0). */
 int ct_hip_set_bp_config(ct_hip_ctx* ctx, int config);

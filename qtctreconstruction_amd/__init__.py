@@ -110,7 +110,7 @@ def load_library(path: str = LIB_PATH):
         "ct_hip_set_filter_backend": (i, [ctx, i]),
         "ct_hip_fbp_batch_dev": (i, [ctx, ctypes.POINTER(FbpJob), i, i]),
         "ct_hip_bp_config_count": (i, []),
-        "ct_hip_bp_config": (i, [i, ctypes.POINTER(i), ctypes.POINTER(i)]),
+        "ct_hip_bp_config": (i, [i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]),
         "ct_hip_autotune": (i, [ctx, i, i, ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double), i]),
         "ct_hip_set_bp_config": (i, [ctx, i]),
         "ct_hip_server_create": (i, [i, sz, i, i, ctypes.POINTER(_vp)]),
@@ -281,12 +281,13 @@ class Context:
         return out
 
     def bp_configs(self):
-        """[(tile, angles_per_stage)] of the instantiated backprojection configurations."""
+        """[(tile, angles_per_stage, slices_per_thread)] of the instantiated
+        backprojection configurations."""
         out = []
         for k in range(self.lib.ct_hip_bp_config_count()):
-            t, a = ctypes.c_int(), ctypes.c_int()
-            self.lib.ct_hip_bp_config(k, ctypes.byref(t), ctypes.byref(a))
-            out.append((t.value, a.value))
+            t, a, z = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            self.lib.ct_hip_bp_config(k, ctypes.byref(t), ctypes.byref(a), ctypes.byref(z))
+            out.append((t.value, a.value, z.value))
         return out
 
     def set_bp_config(self, cfg: int):

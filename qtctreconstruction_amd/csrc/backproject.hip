@@ -1,28 +1,31 @@
 // backproject.hip — voxel-driven backprojection kernels (the tile routine and
 // its numerics are in bp_tile.h).
 //
-//   bp_kernel<T,A,CHECK>          one volume, grid (tiles_x, tiles_y, z-blocks)
-//   bp_persistent_kernel<T,A>     many volumes of uneven size in one launch:
-//                                 work items (job, z-block, tile) are taken from
-//                                 an atomic counter until the list is exhausted
+//   bp_kernel<T,A,CHECK,Z>        one volume, grid (tiles_x, tiles_y, z-block
+//                                 groups of Z)
+//   bp_persistent_kernel<T,A,Z>   many volumes of uneven size in one launch:
+//                                 work items (job, z-block group, tile) are
+//                                 taken from an atomic counter until the list
+//                                 is exhausted
 //
-// (T, A) = (voxel tile edge, angles per LDS stage) is a tuning choice; the
-// instantiated set is kBpConfigs and the autotuner picks among them.
+// (T, A, Z) = (voxel tile edge, angles per LDS stage, z-blocks per tile) is a
+// tuning choice; the instantiated set is kBpConfigs and the autotuner picks
+// among them.
 #include "bp_tile.h"
 
 namespace cthip {
 
-template <int T, int A, bool CHECK>
+template <int T, int A, bool CHECK, int Z>
 __global__ __launch_bounds__(256) void bp_kernel(BpArgs p) {
-    __shared__ BpSmem<T, A> sm;
-    bp_tile<T, A, CHECK>(p, blockIdx.x * T, blockIdx.y * T, blockIdx.z, sm);
+    __shared__ BpSmem<T, A, Z> sm;
+    bp_tile<T, A, CHECK, Z>(p, blockIdx.x * T, blockIdx.y * T, blockIdx.z, sm);
 }
 
-template <int T, int A>
+template <int T, int A, int Z>
 __global__ __launch_bounds__(256) void bp_persistent_kernel(const BpJob* __restrict__ jobs,
                                                             const int* __restrict__ hdr,
                                                             int* __restrict__ counter) {
-    __shared__ BpSmem<T, A> sm;
+    __shared__ BpSmem<T, A, Z> sm;
     __shared__ int item_s;
     const int njobs = hdr[0], total = hdr[1];
     for (;;) {
@@ -37,13 +40,13 @@ __global__ __launch_bounds__(256) void bp_persistent_kernel(const BpJob* __restr
         const int tx = (jb.a.N + T - 1) / T;
         const int per = tx * tx;
         int local = item - jb.item_begin;
-        const int zb = local / per;
-        local -= zb * per;
+        const int zp = local / per;  // z-block group
+        local -= zp * per;
         const int by = local / tx, bx = local - by * tx;
         if (jb.a.check)
-            bp_tile<T, A, true>(jb.a, bx * T, by * T, zb, sm);
+            bp_tile<T, A, true, Z>(jb.a, bx * T, by * T, zp, sm);
         else
-            bp_tile<T, A, false>(jb.a, bx * T, by * T, zb, sm);
+            bp_tile<T, A, false, Z>(jb.a, bx * T, by * T, zp, sm);
     }
 }
 
@@ -52,26 +55,30 @@ __global__ void zero_ints_kernel(int* p, int n) {
     if (i < n) p[i] = 0;
 }
 
-const BpConfig kBpConfigs[kNumBpConfigs] = {{32, 32}, {32, 16}, {32, 64}, {16, 32}, {16, 64}};
+const BpConfig kBpConfigs[kNumBpConfigs] = {{32, 32, 1}, {32, 16, 1}, {32, 64, 1}, {16, 32, 1},
+                                            {16, 64, 1}, {32, 8, 2},  {32, 16, 2}, {16, 16, 2}};
 
-template <int T, int A>
+template <int T, int A, int Z>
 static hipError_t launch_one(const BpArgs& p, hipStream_t s) {
     const int t = (p.N + T - 1) / T;
-    dim3 grid(t, t, (p.nz + kZB - 1) / kZB);
+    dim3 grid(t, t, ((p.nz + kZB - 1) / kZB + Z - 1) / Z);
     if (p.check)
-        hipLaunchKernelGGL((bp_kernel<T, A, true>), grid, dim3(256), 0, s, p);
+        hipLaunchKernelGGL((bp_kernel<T, A, true, Z>), grid, dim3(256), 0, s, p);
     else
-        hipLaunchKernelGGL((bp_kernel<T, A, false>), grid, dim3(256), 0, s, p);
+        hipLaunchKernelGGL((bp_kernel<T, A, false, Z>), grid, dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
 hipError_t launch_backproject_cfg(int cfg, const BpArgs& p, hipStream_t s) {
     switch (cfg) {
-        case 0: return launch_one<32, 32>(p, s);
-        case 1: return launch_one<32, 16>(p, s);
-        case 2: return launch_one<32, 64>(p, s);
-        case 3: return launch_one<16, 32>(p, s);
-        case 4: return launch_one<16, 64>(p, s);
+        case 0: return launch_one<32, 32, 1>(p, s);
+        case 1: return launch_one<32, 16, 1>(p, s);
+        case 2: return launch_one<32, 64, 1>(p, s);
+        case 3: return launch_one<16, 32, 1>(p, s);
+        case 4: return launch_one<16, 64, 1>(p, s);
+        case 5: return launch_one<32, 8, 2>(p, s);
+        case 6: return launch_one<32, 16, 2>(p, s);
+        case 7: return launch_one<16, 16, 2>(p, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -94,18 +101,21 @@ hipError_t launch_backproject(const float4* d_filt, int nz, int na, int sq, int 
     return launch_backproject_cfg(cfg, p, s);
 }
 
-template <int T, int A>
+template <int T, int A, int Z>
 static const void* persistent_fn() {
-    return (const void*)bp_persistent_kernel<T, A>;
+    return (const void*)bp_persistent_kernel<T, A, Z>;
 }
 
 static const void* bp_persistent_ptr(int cfg) {
     switch (cfg) {
-        case 0: return persistent_fn<32, 32>();
-        case 1: return persistent_fn<32, 16>();
-        case 2: return persistent_fn<32, 64>();
-        case 3: return persistent_fn<16, 32>();
-        case 4: return persistent_fn<16, 64>();
+        case 0: return persistent_fn<32, 32, 1>();
+        case 1: return persistent_fn<32, 16, 1>();
+        case 2: return persistent_fn<32, 64, 1>();
+        case 3: return persistent_fn<16, 32, 1>();
+        case 4: return persistent_fn<16, 64, 1>();
+        case 5: return persistent_fn<32, 8, 2>();
+        case 6: return persistent_fn<32, 16, 2>();
+        case 7: return persistent_fn<16, 16, 2>();
         default: return nullptr;
     }
 }
@@ -123,11 +133,14 @@ int bp_persistent_grid(int cfg, int device) {
 hipError_t launch_bp_persistent(int cfg, const BpJob* d_jobs, const int* d_hdr, int* d_counter,
                                 int grid, hipStream_t s) {
     switch (cfg) {
-        case 0: hipLaunchKernelGGL((bp_persistent_kernel<32, 32>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
-        case 1: hipLaunchKernelGGL((bp_persistent_kernel<32, 16>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
-        case 2: hipLaunchKernelGGL((bp_persistent_kernel<32, 64>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
-        case 3: hipLaunchKernelGGL((bp_persistent_kernel<16, 32>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
-        case 4: hipLaunchKernelGGL((bp_persistent_kernel<16, 64>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 0: hipLaunchKernelGGL((bp_persistent_kernel<32, 32, 1>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 1: hipLaunchKernelGGL((bp_persistent_kernel<32, 16, 1>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 2: hipLaunchKernelGGL((bp_persistent_kernel<32, 64, 1>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 3: hipLaunchKernelGGL((bp_persistent_kernel<16, 32, 1>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 4: hipLaunchKernelGGL((bp_persistent_kernel<16, 64, 1>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 5: hipLaunchKernelGGL((bp_persistent_kernel<32, 8, 2>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 6: hipLaunchKernelGGL((bp_persistent_kernel<32, 16, 2>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
+        case 7: hipLaunchKernelGGL((bp_persistent_kernel<16, 16, 2>), dim3(grid), dim3(256), 0, s, d_jobs, d_hdr, d_counter); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -215,7 +215,7 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
     }
     const int cfg = (flags & CT_BATCH_AUTOTUNE) ? autotune(c, jobs[dom].bins, jobs[dom].na, nullptr)
                                                 : c->bp_config;
-    const int T = kBpConfigs[cfg].tile;
+    const int T = kBpConfigs[cfg].tile, Z2 = kBpConfigs[cfg].zb2;
     std::vector<int> tiles(nj);
     std::vector<BpJob> bj(nj);
     std::vector<FiltJob> fj(nj);
@@ -251,7 +251,7 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
         const int i = order[k];
         s->h_btab[k] = bj[i];
         s->h_btab[k].item_begin = bitems;
-        bitems += ((jobs[i].nz + kZB - 1) / kZB) * tiles[i];
+        bitems += (((jobs[i].nz + kZB - 1) / kZB + Z2 - 1) / Z2) * tiles[i];
     }
     // Filter work lists, one per FFT length: a persistent launch reserves LDS
     // for its largest transform, so lengths are not mixed in one launch.
@@ -385,10 +385,11 @@ int ct_hip_autotune(ct_hip_ctx* c, int N, int na, int* config_out, double* ms_ou
 
 int ct_hip_bp_config_count(void) { return kNumBpConfigs; }
 
-int ct_hip_bp_config(int i, int* tile, int* angles) {
+int ct_hip_bp_config(int i, int* tile, int* angles, int* slices) {
     if (i < 0 || i >= kNumBpConfigs || !tile || !angles) return CT_ERR_INVALID;
     *tile = kBpConfigs[i].tile;
     *angles = kBpConfigs[i].angles;
+    if (slices) *slices = kZB * kBpConfigs[i].zb2;
     return CT_OK;
 }
 

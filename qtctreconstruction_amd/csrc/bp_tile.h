@@ -11,8 +11,11 @@
 // to Serial's; the interpolation is one FMA f[i0] + fr*(f[i0+1]-f[i0]) and
 // the angle sum runs in Serial's order.
 //
-// A tile is T x T voxels (T = 16 or 32) of one z-block (4 slices); its 256
-// threads form a 16 x 16 grid and each owns (T/16)^2 voxels x 4 slices.
+// A tile is T x T voxels (T = 16 or 32) of ZB2 z-blocks (4 slices each,
+// ZB2 = 1 or 2); its 256 threads form a 16 x 16 grid and each owns (T/16)^2
+// voxels x 4*ZB2 slices, so one voxel's detector position and fraction serve
+// 4*ZB2 slices. Each slice's sum is the same sequence of operations whatever
+// T, A and ZB2, so every configuration gives bit-identical volumes.
 // Angles go in stages of A: per stage the block stages into LDS, for every
 // angle, the detector window the tile can touch (F bins from
 // floor(min corner u) - 1; u is monotone in x and y, so the corners bound it)
@@ -28,10 +31,10 @@ namespace cthip {
 // detector bins one angle of a T-tile can touch: ceil((T-1)*sqrt2) + 4
 __host__ __device__ constexpr int bp_window(int T) { return T == 32 ? 48 : (T == 16 ? 26 : 0); }
 
-template <int T, int A>
+template <int T, int A, int ZB2>
 struct BpSmem {
-    float4 f[A][bp_window(T)];
-    float4 d[A][bp_window(T)];
+    float4 f[ZB2][A][bp_window(T)];
+    float4 d[ZB2][A][bp_window(T)];
     float2 cs[A];  // (cos, sin) of the stage's angles
     int lo[A];
 };
@@ -45,9 +48,9 @@ typedef const f4v __attribute__((address_space(1))) g_cfloat4;
 typedef float __attribute__((address_space(1))) g_float;
 typedef const float __attribute__((address_space(1))) g_cfloat;
 
-template <int T, int A, bool CHECK>
-__device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
-                                        BpSmem<T, A>& sm) {
+template <int T, int A, bool CHECK, int ZB2>
+__device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zp,
+                                        BpSmem<T, A, ZB2>& sm) {
     constexpr int F = bp_window(T);
     constexpr int V = T / 16;  // voxels per thread per axis
     const int tid = threadIdx.x;
@@ -62,12 +65,23 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
     }
     const float cxa = (float)bx - R, cxb = (float)(bx + T - 1) - R;
     const float cya = (float)by - R, cyb = (float)(by + T - 1) - R;
-    float4 acc[V][V];
+    float4 acc[ZB2][V][V];
 #pragma unroll
-    for (int i = 0; i < V; ++i)
+    for (int q = 0; q < ZB2; ++q)
 #pragma unroll
-        for (int j = 0; j < V; ++j) acc[i][j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    g_cfloat4* __restrict__ fz = (g_cfloat4*)p.filt + (size_t)zb * p.na * p.sq;
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+            for (int j = 0; j < V; ++j) acc[q][i][j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // z-blocks zp*ZB2 + q; one past the last z-block stages zeros
+    const int nzb = (p.nz + kZB - 1) / kZB;
+    g_cfloat4* fz[ZB2];
+    bool zok[ZB2];
+#pragma unroll
+    for (int q = 0; q < ZB2; ++q) {
+        const int zb = zp * ZB2 + q;
+        zok[q] = zb < nzb;
+        fz[q] = (g_cfloat4*)p.filt + (size_t)min(zb, nzb - 1) * p.na * p.sq;
+    }
     g_cfloat* __restrict__ cos_t = (g_cfloat*)p.cos_t;
     g_cfloat* __restrict__ sin_t = (g_cfloat*)p.sin_t;
 
@@ -86,19 +100,23 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
         for (int e = tid; e < nc * F; e += 256) {
             const int aa = e / F, j = e - aa * F;
             const int g = sm.lo[aa] + j;
-            g_cfloat4* row = fz + (size_t)(a0 + aa) * p.sq;
-            const f4v z4 = {0.f, 0.f, 0.f, 0.f};
-            const f4v f0 = (g >= 0 && g < p.sq) ? row[g] : z4;
-            const f4v f1 = (g + 1 >= 0 && g + 1 < p.sq) ? row[g + 1] : z4;
-            sm.f[aa][j] = make_float4(f0.x, f0.y, f0.z, f0.w);
-            sm.d[aa][j] = make_float4(f1.x - f0.x, f1.y - f0.y, f1.z - f0.z, f1.w - f0.w);
+            const bool in0 = g >= 0 && g < p.sq, in1 = g + 1 >= 0 && g + 1 < p.sq;
+#pragma unroll
+            for (int q = 0; q < ZB2; ++q) {
+                g_cfloat4* row = fz[q] + (size_t)(a0 + aa) * p.sq;
+                const f4v z4 = {0.f, 0.f, 0.f, 0.f};
+                const f4v f0 = (zok[q] && in0) ? row[g] : z4;
+                const f4v f1 = (zok[q] && in1) ? row[g + 1] : z4;
+                sm.f[q][aa][j] = make_float4(f0.x, f0.y, f0.z, f0.w);
+                sm.d[q][aa][j] = make_float4(f1.x - f0.x, f1.y - f0.y, f1.z - f0.z, f1.w - f0.w);
+            }
         }
         __syncthreads();
         for (int aa = 0; aa < nc; ++aa) {
             const float2 cs = sm.cs[aa];
             const float c = cs.x, s = cs.y;
             // byte offset of this angle's window entry for bin 0: entry
-            // i0 - lo of row aa sits at i0*16 + boff
+            // i0 - lo of row aa of z-block q sits at i0*16 + boff + q*A*F*16
             const int boff = __builtin_amdgcn_readfirstlane((aa * F - sm.lo[aa]) * 16);
             float xc[V], ys[V];
 #pragma unroll
@@ -129,15 +147,18 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
                     // floor(u) lies between the tile corners' floors, at most
                     // ceil((T-1)*sqrt2) apart (truncation adds at most 1)
                     const int bo = i0 * 16 + boff;
-                    const float4 f = *reinterpret_cast<const float4*>(
-                        reinterpret_cast<const char*>(&sm.f[0][0]) + bo);
-                    const float4 d = *reinterpret_cast<const float4*>(
-                        reinterpret_cast<const char*>(&sm.d[0][0]) + bo);
-                    if (!CHECK || (i0 >= 0 && i0 + 1 < p.sq)) {
-                        acc[i][j].x += fmaf(fr, d.x, f.x);
-                        acc[i][j].y += fmaf(fr, d.y, f.y);
-                        acc[i][j].z += fmaf(fr, d.z, f.z);
-                        acc[i][j].w += fmaf(fr, d.w, f.w);
+#pragma unroll
+                    for (int q = 0; q < ZB2; ++q) {
+                        const float4 f = *reinterpret_cast<const float4*>(
+                            reinterpret_cast<const char*>(&sm.f[q][0][0]) + bo);
+                        const float4 d = *reinterpret_cast<const float4*>(
+                            reinterpret_cast<const char*>(&sm.d[q][0][0]) + bo);
+                        if (!CHECK || (i0 >= 0 && i0 + 1 < p.sq)) {
+                            acc[q][i][j].x += fmaf(fr, d.x, f.x);
+                            acc[q][i][j].y += fmaf(fr, d.y, f.y);
+                            acc[q][i][j].z += fmaf(fr, d.z, f.z);
+                            acc[q][i][j].w += fmaf(fr, d.w, f.w);
+                        }
                     }
                 }
         }
@@ -145,37 +166,42 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zb,
     }
 
     const size_t ss = (size_t)p.N * p.N;
-    float lo_z[kZB], eff_z[kZB];
 #pragma unroll
-    for (int k = 0; k < kZB; ++k) {
-        const int z = zb * kZB + k;
-        lo_z[k] = 0.0f;
-        eff_z[k] = 1.0f;
-        if (p.mm && z < p.nz) {
-            g_cfloat* mm = (g_cfloat*)p.mm;
-            const float span = mm[2 * z + 1] - mm[2 * z];
-            lo_z[k] = mm[2 * z];
-            eff_z[k] = (span <= 0.0f) ? 1.0f : span;  // SerialBackend.cpp:298-304
-        }
-    }
+    for (int q = 0; q < ZB2; ++q) {
+        const int zb = zp * ZB2 + q;
+        float lo_z[kZB], eff_z[kZB];
 #pragma unroll
-    for (int i = 0; i < V; ++i)
-#pragma unroll
-        for (int j = 0; j < V; ++j) {
-            const int x = bx + tx + 16 * i, y = by + ty + 16 * j;
-            if (x >= p.N || y >= p.N) continue;
-            const float xf = (float)x - R, yf = (float)y - R;
-            const bool inside = !(xf * xf + yf * yf > R * R);
-            const float v4[4] = {acc[i][j].x, acc[i][j].y, acc[i][j].z, acc[i][j].w};
-#pragma unroll
-            for (int k = 0; k < kZB; ++k) {
-                const int z = zb * kZB + k;
-                if (z >= p.nz) break;
-                float v = inside ? v4[k] * p.scale : 0.0f;
-                if (p.mm) v = v * eff_z[k] + lo_z[k];
-                ((g_float*)p.out)[(size_t)z * ss + (size_t)y * p.N + x] = v;
+        for (int k = 0; k < kZB; ++k) {
+            const int z = zb * kZB + k;
+            lo_z[k] = 0.0f;
+            eff_z[k] = 1.0f;
+            if (p.mm && z < p.nz) {
+                g_cfloat* mm = (g_cfloat*)p.mm;
+                const float span = mm[2 * z + 1] - mm[2 * z];
+                lo_z[k] = mm[2 * z];
+                eff_z[k] = (span <= 0.0f) ? 1.0f : span;  // SerialBackend.cpp:298-304
             }
         }
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+            for (int j = 0; j < V; ++j) {
+                const int x = bx + tx + 16 * i, y = by + ty + 16 * j;
+                if (x >= p.N || y >= p.N) continue;
+                const float xf = (float)x - R, yf = (float)y - R;
+                const bool inside = !(xf * xf + yf * yf > R * R);
+                const float v4[4] = {acc[q][i][j].x, acc[q][i][j].y, acc[q][i][j].z,
+                                     acc[q][i][j].w};
+#pragma unroll
+                for (int k = 0; k < kZB; ++k) {
+                    const int z = zb * kZB + k;
+                    if (z >= p.nz) break;
+                    float v = inside ? v4[k] * p.scale : 0.0f;
+                    if (p.mm) v = v * eff_z[k] + lo_z[k];
+                    ((g_float*)p.out)[(size_t)z * ss + (size_t)y * p.N + x] = v;
+                }
+            }
+    }
 }
 
 }  // namespace cthip

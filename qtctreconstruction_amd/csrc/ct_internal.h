@@ -126,14 +126,15 @@ struct BpArgs {
 };
 struct BpJob {
     BpArgs a;
-    int item_begin;  // first work item; items are (z-block, tile) in z-major order
+    int item_begin;  // first work item; items are (z-block group, tile) in z-major order
     int pad_;
 };
 struct BpConfig {
     int tile;    // voxel tile edge (16 or 32)
     int angles;  // angles per LDS stage
+    int zb2;     // z-blocks (of 4 slices) per tile: 1 or 2
 };
-constexpr int kNumBpConfigs = 5;
+constexpr int kNumBpConfigs = 8;
 extern const BpConfig kBpConfigs[kNumBpConfigs];
 // out[z][y][x] = inside ? acc*scale : 0, then (if d_mm) * eff_span + min.
 hipError_t launch_backproject(const float4* d_filt, int nz, int na, int sq, int N,
