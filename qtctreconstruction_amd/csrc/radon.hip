@@ -22,7 +22,6 @@
 #include "ct_internal.h"
 
 #include <algorithm>
-#include <cstdlib>
 
 namespace cthip {
 
@@ -344,14 +343,30 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
 // read the row-major copy. The others enter through a side edge and are on
 // consecutive rows of one common column at every step: they read the
 // column-major copy. Either way a wave's loads are close to contiguous.
+//
+// A launch covers kXcds / gsz z-blocks from zb0 (nzl of them present). Work
+// groups are dispatched to the XCDs round-robin (u -> XCD u % kXcds), so
+// workgroup u takes z-block (u % kXcds) / gsz and the (bin block, angle)
+// slot (u / kXcds) * gsz + (u % kXcds) % gsz: each z-block is walked by one
+// group of gsz XCDs and its lines stay in their L2s. (The mapping covers
+// every (z-block, slot) once whatever the dispatch order; only the speed
+// depends on it.)
+constexpr int kXcds = 8;
 template <int U>
 __global__ __launch_bounds__(256) void radon_ray_kernel(
     const float4* __restrict__ rowL, const float4* __restrict__ colL, int w, int h, int nz,
     int na, int bins, const float* __restrict__ cos_t, const float* __restrict__ sin_t,
-    const int* __restrict__ alist, float* __restrict__ sino, int zb0) {
-    const int i = blockIdx.x * 256 + threadIdx.x;  // bin owned by this lane
-    const int a = alist[blockIdx.y];
-    const int zb = zb0 + blockIdx.z;
+    const int* __restrict__ alist, float* __restrict__ sino, int zb0, int nzl, int gsz,
+    int n_dir) {
+    const int u = blockIdx.x;
+    const int xcd = u % kXcds;
+    const int zl = xcd / gsz;
+    const int bb = (bins + 255) / 256;
+    const int slot = (u / kXcds) * gsz + xcd % gsz;
+    if (zl >= nzl || slot >= bb * n_dir) return;
+    const int i = (slot % bb) * 256 + threadIdx.x;  // bin owned by this lane
+    const int a = alist[slot / bb];
+    const int zb = zb0 + zl;
     if (i >= bins) return;
     int y0, y1;
     const RayGeom g = ray_geom(i, bins, w, h, cos_t[a], sin_t[a], y0, y1);
@@ -546,23 +561,22 @@ hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, 
     if (n_dir <= 0) return hipSuccess;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int bb = (bins + 255) / 256;
-    // At least ~720 blocks per launch, as few z-blocks as that allows (two at
-    // 512 bins x 359 angles, which measured faster than one): the blocks of one
-    // launch start together and sweep the same copy in step, so most of a
-    // z-block's pixels come from HBM once per launch rather than once per
-    // angle.
+    // XCDs per z-block: about 300 blocks of the z-block per XCD and launch,
+    // about 1.5 times what an XCD holds at once (two XCDs per z-block at 512
+    // bins x 295 angles, four at 1024 bins: the best of 1, 2, 4 and 8 at
+    // both). The blocks of one launch start together and sweep their
+    // z-block's copies in step, so most pixels come from HBM once per launch.
     const int zbb = bb * n_dir;
-    int per = std::max(1, (720 + zbb - 1) / zbb);
-    if (const char* env = std::getenv("CT_HIP_RADON_STEEP_ZB")) {
-        const int v = std::atoi(env);
-        if (v >= 1) per = v;
-    }
+    int gsz = 1;
+    while (gsz < kXcds && zbb / (2 * gsz) >= 225) gsz *= 2;
+    const int per = kXcds / gsz;
+    const int slots = (zbb + gsz - 1) / gsz;  // workgroups per XCD
     for (int zb0 = 0; zb0 < nzb; zb0 += per) {
         // chunks of 4 candidates (8 loads in flight per lane); smaller
         // chunks load fewer pixels past the runs but measured slower
-        hipLaunchKernelGGL(radon_ray_kernel<4>, dim3(bb, n_dir, std::min(per, nzb - zb0)),
-                           dim3(256), 0, s, d_row, d_col, w, h, nz, na, bins, d_cos, d_sin,
-                           d_alist, d_sino, zb0);
+        hipLaunchKernelGGL(radon_ray_kernel<4>, dim3(kXcds * slots), dim3(256), 0, s, d_row,
+                           d_col, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, zb0,
+                           std::min(per, nzb - zb0), gsz, n_dir);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (launches) ++*launches;
