@@ -3,6 +3,8 @@
 // context, workspaces and orchestration.
 #pragma once
 
+#include <algorithm>
+
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -24,6 +26,50 @@ constexpr int kPadX = 4;
 // 32 bytes (8 slices)
 __host__ __device__ inline size_t radon_zbytes(int w, int h) {
     return (size_t)(w + kPadX) * h * 32;
+}
+
+// XCD-grouped work order. Workgroups are dispatched to the kXcds XCDs
+// round-robin (workgroup u to XCD u % kXcds), each XCD with its own L2. A
+// launch of kXcds * slots * rounds workgroups covers nunit units (z-blocks)
+// of nslot workgroups each: in round q, units q*P .. q*P + P - 1 (P =
+// kXcds / gsz) are each walked by one group of gsz XCDs, slots = ceil(nslot
+// / gsz) workgroups per XCD, so a unit's data stays in its group's L2s. The
+// mapping is a bijection onto the (unit, slot) pairs whatever the dispatch
+// order; only the speed depends on it.
+constexpr int kXcds = 8;
+__device__ __forceinline__ bool xcd_map(int u, int gsz, int slots, int nslot, int nunit,
+                                        int& unit, int& slot) {
+    const int xcd = u % kXcds, r = u / kXcds;
+    const int round = r / slots, rs = r - round * slots;
+    unit = round * (kXcds / gsz) + xcd / gsz;
+    slot = rs * gsz + xcd % gsz;
+    return unit < nunit && slot < nslot;
+}
+// XCDs per unit for nunit units of nslot workgroups. Each round takes
+// ceil(nslot / g) workgroups per XCD and there are ceil(nunit / (kXcds / g))
+// rounds; among the g within 2% of the fewest workgroups per XCD over all
+// rounds (so few units still fill every XCD), the one nearest ~300 per XCD
+// and round (about 1.5 times what an XCD holds at once, measured best for
+// the Radon walks), the smaller g on a tie.
+inline int xcd_group(int nslot, int nunit) {
+    auto load = [&](int g) {
+        const long long P = kXcds / g;
+        return (double)((nunit + P - 1) / P) * (double)((nslot + g - 1) / g);
+    };
+    double best = 1e300;
+    for (int g = 1; g <= kXcds; g *= 2) best = std::min(best, load(g));
+    int pick = kXcds;
+    double dist = 1e300;
+    for (int g = 1; g <= kXcds; g *= 2) {
+        if (load(g) > best * 1.02) continue;
+        const double per = (double)((nslot + g - 1) / g);
+        const double d = per > 300.0 ? per / 300.0 : 300.0 / per;
+        if (d < dist) {
+            dist = d;
+            pick = g;
+        }
+    }
+    return pick;
 }
 
 // Largest FFT length of the LDS row filter (bins <= 1448).

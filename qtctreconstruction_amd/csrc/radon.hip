@@ -344,26 +344,19 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
 // consecutive rows of one common column at every step: they read the
 // column-major copy. Either way a wave's loads are close to contiguous.
 //
-// A launch covers kXcds / gsz z-blocks from zb0 (nzl of them present). Work
-// groups are dispatched to the XCDs round-robin (u -> XCD u % kXcds), so
-// workgroup u takes z-block (u % kXcds) / gsz and the (bin block, angle)
-// slot (u / kXcds) * gsz + (u % kXcds) % gsz: each z-block is walked by one
-// group of gsz XCDs and its lines stay in their L2s. (The mapping covers
-// every (z-block, slot) once whatever the dispatch order; only the speed
-// depends on it.)
-constexpr int kXcds = 8;
+// A launch is one round of the XCD-grouped order (xcd_map): kXcds / gsz
+// z-blocks from zb0 (nzl of them present), each walked by one group of gsz
+// XCDs; a z-block's slots are its (bin block, angle) pairs.
 template <int U>
 __global__ __launch_bounds__(256) void radon_ray_kernel(
     const float4* __restrict__ rowL, const float4* __restrict__ colL, int w, int h, int nz,
     int na, int bins, const float* __restrict__ cos_t, const float* __restrict__ sin_t,
     const int* __restrict__ alist, float* __restrict__ sino, int zb0, int nzl, int gsz,
     int n_dir) {
-    const int u = blockIdx.x;
-    const int xcd = u % kXcds;
-    const int zl = xcd / gsz;
     const int bb = (bins + 255) / 256;
-    const int slot = (u / kXcds) * gsz + xcd % gsz;
-    if (zl >= nzl || slot >= bb * n_dir) return;
+    const int slots = (bb * n_dir + gsz - 1) / gsz;
+    int zl, slot;
+    if (!xcd_map(blockIdx.x, gsz, slots, bb * n_dir, nzl, zl, slot)) return;
     const int i = (slot % bb) * 256 + threadIdx.x;  // bin owned by this lane
     const int a = alist[slot / bb];
     const int zb = zb0 + zl;
@@ -402,20 +395,26 @@ __global__ __launch_bounds__(256) void radon_ray_kernel(
 // its bin and add +0.
 constexpr int kRadonRows = 2;  // rows staged per step
 
+// Workgroups follow the XCD-grouped order (xcd_map): units are z-blocks,
+// a z-block's slots its (bin block, angle) pairs, n_ang angles.
 template <int RR, int U, int kRowW>
 __global__ __launch_bounds__(256) void radon_rows_kernel(
     const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
     const float* __restrict__ cos_t, const float* __restrict__ sin_t,
-    const int* __restrict__ alist, float* __restrict__ sino) {
+    const int* __restrict__ alist, float* __restrict__ sino, int n_ang, int gsz) {
     // window: lanes 255 bins apart are at most 255/|cos| pixels apart, plus a
     // chunk of U; the window must also keep every chunk inside the zero pad
     static_assert(U - 2 <= kPadX - 1, "chunk reaches past the zero pad");
     __shared__ float4 t0[RR][kRowW];  // slices 0-3 of the staged pixels
     __shared__ float4 t1[RR][kRowW];  // slices 4-7
     __shared__ int xw_s[2][RR];       // window starts: staged rows, next rows
-    const int a = alist[blockIdx.y];
-    const int zb = blockIdx.z;
-    const int b0 = blockIdx.x * 256;
+    const int bb = (bins + 255) / 256;
+    const int nzb = (nz + kRZ - 1) / kRZ;
+    const int slots = (bb * n_ang + gsz - 1) / gsz;
+    int zb, slot;
+    if (!xcd_map(blockIdx.x, gsz, slots, bb * n_ang, nzb, zb, slot)) return;  // whole block
+    const int a = alist[slot / bb];
+    const int b0 = (slot % bb) * 256;
     const int i = b0 + threadIdx.x;
     const bool live = i < bins;
     const float c = cos_t[a], sn = sin_t[a];
@@ -536,17 +535,24 @@ hipError_t launch_radon_rows(const float4* d_row, int w, int h, int nz, int na, 
                              int n_row3, int n_row4, float* d_sino, hipStream_t s) {
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int bb = (bins + 255) / 256;
+    // workgroups of the XCD-grouped order for n angles
+    auto grid = [&](int n, int& gsz) {
+        gsz = xcd_group(bb * n, nzb);
+        const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
+        return dim3((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
+    };
+    int gsz = 1;
     if (n_row3 > 0) {  // |sin| <= |cos|: 3 candidates, window 368
-        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 368>), dim3(bb, n_row3, nzb),
-                           dim3(256), 0, s, d_row, w, h, nz, na, bins, d_cos, d_sin, d_alist,
-                           d_sino);
+        const dim3 g = grid(n_row3, gsz);
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 368>), g, dim3(256), 0, s, d_row, w,
+                           h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n_row3, gsz);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     if (n_row4 > 0) {  // 0.6 <= |cos| < |sin|: 4 candidates, window 432
-        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 4, 432>), dim3(bb, n_row4, nzb),
-                           dim3(256), 0, s, d_row, w, h, nz, na, bins, d_cos, d_sin,
-                           d_alist + n_row3, d_sino);
+        const dim3 g = grid(n_row4, gsz);
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 4, 432>), g, dim3(256), 0, s, d_row, w,
+                           h, nz, na, bins, d_cos, d_sin, d_alist + n_row3, d_sino, n_row4, gsz);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -567,8 +573,7 @@ hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, 
     // both). The blocks of one launch start together and sweep their
     // z-block's copies in step, so most pixels come from HBM once per launch.
     const int zbb = bb * n_dir;
-    int gsz = 1;
-    while (gsz < kXcds && zbb / (2 * gsz) >= 225) gsz *= 2;
+    const int gsz = xcd_group(zbb, nzb);
     const int per = kXcds / gsz;
     const int slots = (zbb + gsz - 1) / gsz;  // workgroups per XCD
     for (int zb0 = 0; zb0 < nzb; zb0 += per) {
