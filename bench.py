@@ -38,7 +38,6 @@ FLOP_PER_SAMPLE = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_ray_kernel": 1.422e+09,
     "radon_rows_kernel": 2.067e+10,
     "bp_kernel": 6.534e+09,
 }
@@ -72,43 +71,31 @@ def cpu_baseline(vol_host: np.ndarray, na: int, ftype: int, target_s: float):
                       f"{dt:.2f} s wall"}
 
 
-def row_walk_angles(na: int) -> int:
-    """Angles of the Serial set the Radon row walk takes: |sin| <= |cos| or
-    |cos| >= 0.6 (float32 trig, as the host tables)."""
-    a = np.arange(na, dtype=np.float32) * np.float32(180.0 / na)
-    th = (a * np.float32(3.14159265359)) / np.float32(180.0)
-    c = np.cos(th.astype(np.float64)).astype(np.float32)
-    s = np.sin(th.astype(np.float64)).astype(np.float32)
-    return int(np.count_nonzero(~(np.abs(s) > np.abs(c)) | (np.abs(c) >= np.float32(0.6))))
-
-
-def dominant_roofline(n, na, stage, radon_k, dir_launches):
+def dominant_roofline(n, na, stage, radon_k, launches):
     """Roofline entry of the kernel with the largest time per step.
 
-    Candidates: the Radon direct walk (radon_ray_kernel, several launches),
-    the Radon row walk (radon_rows_kernel), the backprojection (bp_kernel)
-    and the FFT row filter (row_filter_fft_kernel). achieved = algorithmic
-    FLOP per launch / average launch time (HIP events on the context stream).
+    Candidates: the Radon row walk (radon_rows_kernel: one launch over the
+    image for the angles with |sin| <= |cos|, one over the transposed image
+    for the others), the backprojection (bp_kernel) and the FFT row filter.
+    achieved = algorithmic FLOP per step / the kernel's time per step (HIP
+    events on the context stream), over its launches per step.
     """
-    n_row = row_walk_angles(na)
     samples = float(n) * n * n  # pixels x slices per angle
     cands = {
-        "radon_ray_kernel": (radon_k[1], max(dir_launches, 1),
-                             FLOP_PER_SAMPLE * samples * (na - n_row),
-                             "Radon direct walk (angles with |cos| < 0.6)"),
-        "radon_rows_kernel": (radon_k[0], 1, FLOP_PER_SAMPLE * samples * n_row,
-                              "Radon LDS row walk (angles with |cos| >= 0.6)"),
+        "radon_rows_kernel": (radon_k[0] + radon_k[1], 2 * max(launches, 1),
+                              FLOP_PER_SAMPLE * samples * na,
+                              "Radon LDS row walk (image and transposed image)"),
         "bp_kernel": (stage[2], 1, FLOP_PER_UPDATE * samples * na,
                       "voxel-driven backprojection"),
     }
     name = max(cands, key=lambda k: cands[k][0])
-    t_ms, launches, flop, what = cands[name]
-    per_launch_ms = t_ms / launches
-    achieved = flop / launches / (per_launch_ms / 1e3) / 1e12
+    t_ms, nl, flop, what = cands[name]
+    per_launch_ms = t_ms / nl
+    achieved = flop / (t_ms / 1e3) / 1e12
     tb = TRAFFIC_BYTES.get(name)
     return {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": tb, "kernel": name,
-            "launches_per_step": launches, "avg_launch_ms": per_launch_ms,
+            "launches_per_step": nl, "avg_launch_ms": per_launch_ms,
             "note": f"{what}; FP32 VALU class (no MFMA work on this path); "
                     f"{'4 FLOP per pixel-angle-slice' if name.startswith('radon') else '9 FLOP per voxel-update'}"
                     "; 157.3 TF is MI355X's FP32 peak (packed FP32 counted), the kernels are "
@@ -347,8 +334,8 @@ def main():
     for _ in range(args.steps):
         step()
         stage += ctx.timings()
-        r_row, r_dir, dir_launches = ctx.radon_timings()
-        radon_k += (r_row, r_dir)
+        r_img, r_tr, walk_launches = ctx.radon_timings()
+        radon_k += (r_img, r_tr)
     ctx.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
@@ -356,10 +343,10 @@ def main():
     radon_k /= args.steps
     updates = float(nvox) * na
     value = updates / (ms / 1e3)
-    log(f"[bench] {ms:.2f} ms/step; stages radon {stage[0]:.2f} (row walk {radon_k[0]:.2f}, "
-        f"direct walk {radon_k[1]:.2f} in {dir_launches} launches) filter {stage[1]:.2f} "
+    log(f"[bench] {ms:.2f} ms/step; stages radon {stage[0]:.2f} (walk over the image "
+        f"{radon_k[0]:.2f}, over the transpose {radon_k[1]:.2f}) filter {stage[1]:.2f} "
         f"bp {stage[2]:.2f} ms")
-    roofline = dominant_roofline(n, na, stage, radon_k, dir_launches)
+    roofline = dominant_roofline(n, na, stage, radon_k, walk_launches)
     log(f"[bench] roofline {roofline}")
 
     extra = {}
@@ -415,8 +402,7 @@ def main():
         "stages_ms": {"normalise_radon": stage[0], "row_filter": stage[1],
                       "backprojection": stage[2]},
         "sinogram_samples_per_s": float(n) * na * n / (ms / 1e3),
-        "stages_ms_radon": {"row_walk": radon_k[0], "direct_walk": radon_k[1],
-                            "direct_walk_launches": dir_launches},
+        "stages_ms_radon": {"walk_image": radon_k[0], "walk_transpose": radon_k[1]},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -256,12 +256,13 @@ int ct_hip_noisy_log_dev(ct_hip_ctx* ctx, float* d_sino, size_t n, float I0, flo
 int ct_hip_last_timings(const ct_hip_ctx* ctx, double* ms4);
 
 /* Breakdown of the Radon stage of the last volume call that ran one,
- * milliseconds, HIP events on the context's stream: [0] the row walk (angles
- * with |cos| >= 0.6, two launches), [1] the direct walk (the other angles);
- * *direct_launches (optional) gets the direct walk's launch count.
- * No counterpart in the reference: per-kernel timing for profiling and the
- * roofline report. CT_ERR_INVALID before any Radon stage has run. */
-int ct_hip_last_radon_timings(const ct_hip_ctx* ctx, double* ms2, int* direct_launches);
+ * milliseconds, HIP events on the context's stream: [0] the row walk over
+ * the image (angles with |sin| <= |cos|), [1] the row walk over the
+ * transposed image (the other angles); *launches (optional) gets the launch
+ * count of each walk (1). No counterpart in the reference: per-kernel timing
+ * for profiling and the roofline report. CT_ERR_INVALID before any Radon
+ * stage has run. */
+int ct_hip_last_radon_timings(const ct_hip_ctx* ctx, double* ms2, int* launches);
 
 #ifdef __cplusplus
 }

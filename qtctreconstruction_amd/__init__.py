@@ -269,7 +269,8 @@ class Context:
         return np.array(t[:], np.float64)
 
     def radon_timings(self):
-        """(row-walk ms, direct-walk ms, direct-walk launches) of the last Radon stage."""
+        """(ms of the row walk over the image, ms of the row walk over the
+        transposed image, launches per walk) of the last Radon stage."""
         t = (ctypes.c_double * 2)()
         n = ctypes.c_int(0)
         self._call("ct_hip_last_radon_timings", self.handle, t, ctypes.byref(n))

@@ -78,27 +78,21 @@ ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
         cs[a] = std::cos(th);
         sn[a] = std::sin(th);
     }
-    // The Radon stage's angle groups, by the same float tests as the kernels':
-    // |sin| <= |cos| (row walk, 3 candidates), 0.6 <= |cos| < |sin| (row walk,
-    // 4 candidates), the rest (direct walk).
+    // The Radon stage's angle groups: |sin| <= |cos| (row walk over the
+    // image), then the rest (row walk over the transposed image).
     std::vector<int> alist;
     alist.reserve(na);
     auto steep = [&](int a) { return std::fabs(sn[a]) > std::fabs(cs[a]); };
-    auto rowcos = [&](int a) { return std::fabs(cs[a]) >= 0.6f; };
     for (int a = 0; a < na; ++a)
         if (!steep(a)) alist.push_back(a);
-    const int n_row3 = (int)alist.size();
+    const int n_row = (int)alist.size();
     for (int a = 0; a < na; ++a)
-        if (steep(a) && rowcos(a)) alist.push_back(a);
-    const int n_row4 = (int)alist.size() - n_row3;
-    for (int a = 0; a < na; ++a)
-        if (steep(a) && !rowcos(a)) alist.push_back(a);
+        if (steep(a)) alist.push_back(a);
     ct_hip_ctx::Trig t;
     t.cos = static_cast<float*>(dmalloc(c, sizeof(float) * na));
     t.sin = static_cast<float*>(dmalloc(c, sizeof(float) * na));
     t.alist = static_cast<int*>(dmalloc(c, sizeof(int) * na));
-    t.n_row3 = n_row3;
-    t.n_row4 = n_row4;
+    t.n_row = n_row;
     hip_check(hipMemcpyAsync(t.cos, cs.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
               "upload cos table");
     hip_check(hipMemcpyAsync(t.sin, sn.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
@@ -168,7 +162,7 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     unsigned* keys = ensure<unsigned>(c, c->keys, sizeof(unsigned) * 2 * (size_t)nz);
     const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
     float4* row = ensure<float4>(c, c->row, rzb * radon_zbytes(w, h));
-    float4* col = ensure<float4>(c, c->col, rzb * radon_zbytes(w, h));
+    float4* col = ensure<float4>(c, c->col, rzb * radon_zbytes(h, w));
     hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
@@ -177,15 +171,14 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     // the other: side by side on two streams they evict each other's cache
     // lines (measured 24% slower at 1024^3, no gain at 512^3).
     hip_check(hipEventRecord(c->evr[0], c->stream), "event");
-    const int n_rows = c->cur.n_row3 + c->cur.n_row4;
+    const int n_row = c->cur.n_row;
     hip_check(launch_radon_rows(row, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
-                                c->cur.n_row3, c->cur.n_row4, d_sino, c->stream),
+                                n_row, d_sino, c->stream),
               "radon rows");
     hip_check(hipEventRecord(c->evr[1], c->stream), "event");
-    hip_check(launch_radon_direct(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin,
-                                  c->cur.alist + n_rows, na - n_rows, d_sino, c->stream,
-                                  &c->radon_direct_launches),
-              "radon direct");
+    hip_check(launch_radon_rows(col, h, w, nz, na, bins, c->cur.sin, c->cur.cos,
+                                c->cur.alist + n_row, na - n_row, d_sino, c->stream),
+              "radon rows of the transpose");
     hip_check(hipEventRecord(c->evr[2], c->stream), "event");
     c->radon_timed = true;
 }
@@ -800,12 +793,12 @@ int ct_hip_last_timings(const ct_hip_ctx* c, double* ms4) {
     return CT_OK;
 }
 
-int ct_hip_last_radon_timings(const ct_hip_ctx* c, double* ms2, int* direct_launches) {
+int ct_hip_last_radon_timings(const ct_hip_ctx* c, double* ms2, int* launches) {
     if (!c || !ms2) return CT_ERR_INVALID;
     if (!c->radon_timed) return CT_ERR_INVALID;
     ms2[0] = c->radon_ms[0];
     ms2[1] = c->radon_ms[1];
-    if (direct_launches) *direct_launches = c->radon_direct_launches;
+    if (launches) *launches = 1;
     return CT_OK;
 }
 

@@ -19,11 +19,11 @@ namespace cthip {
 constexpr int kZB = 4;
 // Slices per Radon z-block (the normalised copies are [zb][..][..][8]).
 constexpr int kRZ = 8;
-// zero pixels after each row (row layout) and columns after the last column
-// (column layout) of the Radon stage's normalised copies
+// zero pixels after each row of the Radon stage's normalised copies (the row
+// copy and the column copy, which is the transpose's row copy)
 constexpr int kPadX = 4;
-// bytes of one z-block of either normalised copy: (w + kPadX) x h pixels of
-// 32 bytes (8 slices)
+// bytes of one z-block of the row copy of a w x h image: (w + kPadX) x h
+// pixels of 32 bytes (8 slices); radon_zbytes(h, w) for the column copy
 __host__ __device__ inline size_t radon_zbytes(int w, int h) {
     return (size_t)(w + kPadX) * h * 32;
 }
@@ -100,21 +100,17 @@ hipError_t launch_minmax(const float* d_vol, size_t slice_elems, int nz, unsigne
 // writes mm[2*z] = min, mm[2*z+1] = max as floats
 hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hipStream_t s);
 // w = (raw == 0) ? 0 : (raw - min) * (1/span); z-blocked row copy
-// [zb][h][w + kPadX] and column copy [zb][w + kPadX][h], kRZ = 8 slices per
-// pixel (2 float4), kPadX zero pixels past each row / column.
+// [zb][h][w + kPadX] and column copy (the transpose) [zb][w][h + kPadX],
+// kRZ = 8 slices per pixel (2 float4), kPadX zero pixels past each row.
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, hipStream_t s);
-// Radon of the angles alist[0..n_row3) (|sin| <= |cos|) and
-// alist[n_row3..n_row3+n_row4) (0.6 <= |cos| < |sin|): LDS-staged row walk
-hipError_t launch_radon_rows(const float4* d_row, int w, int h, int nz, int na, int bins,
-                             const float* d_cos, const float* d_sin, const int* d_alist,
-                             int n_row3, int n_row4, float* d_sino, hipStream_t s);
-// Radon of the n_dir angles alist[0..n_dir) (|sin| > |cos|): direct walk over
-// both copies, a few z-blocks per launch; *launches gets the launch count
-hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, int h, int nz,
-                               int na, int bins, const float* d_cos, const float* d_sin,
-                               const int* d_alist, int n_dir, float* d_sino, hipStream_t s,
-                               int* launches);
+// Radon of the n angles alist[0..n) over an image copy [zb][h][w + kPadX]
+// walked by rows, |sin| <= |cos| for every angle: the LDS-staged row walk.
+// Steeper angles run it on the column copy with (w, h) and (d_cos, d_sin)
+// exchanged.
+hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, int bins,
+                             const float* d_cos, const float* d_sin, const int* d_alist, int n,
+                             float* d_sino, hipStream_t s);
 
 // ---- row filter ----------------------------------------------------------
 struct FiltArgs {

@@ -74,21 +74,19 @@ struct ct_hip_ctx {
     struct Trig {
         float* cos = nullptr;
         float* sin = nullptr;
-        // angle indices for the Radon stage: n_row3 angles with |sin| <= |cos|,
-        // then n_row4 with 0.6 <= |cos| < |sin| (both walk rows), then the
-        // direct-walk angles
+        // angle indices for the Radon stage: n_row angles with |sin| <= |cos|
+        // (row walk over the image), then the rest (over the transpose)
         int* alist = nullptr;
-        int n_row3 = 0, n_row4 = 0;
+        int n_row = 0;
     };
     std::map<std::vector<uint32_t>, Trig> trig;  // keyed by the angle list's bit pattern
     Trig cur;                                     // table of the current plain call
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double timings[4] = {0, 0, 0, 0};
-    // Radon stage breakdown: events before the row walk, between the two
-    // walks and after the direct walk
+    // Radon stage breakdown: events before the walk over the image, between
+    // the two walks and after the walk over the transpose
     hipEvent_t evr[3] = {nullptr, nullptr, nullptr};
     double radon_ms[2] = {0, 0};
-    int radon_direct_launches = 0;
     bool radon_timed = false;
     int filter_backend = 0;  // CT_FILTER_BACKEND_*
     int bp_config = 0;       // backprojection tile configuration (kBpConfigs index)

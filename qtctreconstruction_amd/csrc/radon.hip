@@ -81,13 +81,13 @@ __global__ void minmax_decode_kernel(const unsigned* keys, int nz, float* mm) {
 
 // block (32,8) handles a 32x32 pixel tile of one Radon z-block (kRZ = 8
 // slices): reads the slices, normalises (SerialBackend.cpp:78-80,104,106),
-// writes the row copy directly and the column copy through an LDS
-// transpose. A pixel is 32 bytes (the 8 slices as 2 float4):
+// writes the row copy directly and the column copy (the transpose) through
+// an LDS transpose. A pixel is 32 bytes (the 8 slices as 2 float4):
 //   row copy     byte ((zb*h + y)*(w + kPadX) + x)*32
-//   column copy  byte ((zb*(w + kPadX) + x)*h + y)*32
-// x runs to w + kPadX - 1: the kPadX pixels past each row (columns past the
-// last column) are zeros, so a ray's fixed-size chunk of candidates may run
-// past the edge of the image and read only zeros.
+//   column copy  byte ((zb*w + x)*(h + kPadX) + y)*32
+// The kPadX pixels past each row of either copy (x >= w in the row copy,
+// y >= h in the column copy) are zeros, so a lane's fixed-size chunk of
+// candidates may run past the edge of the image and read only zeros.
 __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict__ vol, int w,
                                                         int h, int nz,
                                                         const float* __restrict__ mm,
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
     __shared__ float4 tile[32][33][2];
     const int zb = blockIdx.z;
     const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
-    const int wp = w + kPadX;
+    const int wp = w + kPadX, hp = h + kPadX;
     float mn[kRZ], inv[kRZ];
     bool live[kRZ];
 #pragma unroll
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
     }
     const size_t ss = (size_t)w * h;
     float4* __restrict__ r0 = rowL + (size_t)zb * h * wp * 2;
-    float4* __restrict__ c0 = colL + (size_t)zb * h * wp * 2;
+    float4* __restrict__ c0 = colL + (size_t)zb * w * hp * 2;
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int x = bx + threadIdx.x, y = by + r;
         float t[kRZ];
@@ -140,8 +140,8 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
     __syncthreads();
     for (int r = threadIdx.y; r < 32; r += 8) {
         const int y = by + threadIdx.x, x = bx + r;
-        if (x < wp && y < h) {
-            float4* d = c0 + ((size_t)x * h + y) * 2;
+        if (x < w && y < hp) {  // pixels h..h+kPadX-1 of a column are the zero pad
+            float4* d = c0 + ((size_t)x * hp + y) * 2;
             d[0] = tile[threadIdx.x][r][0];
             d[1] = tile[threadIdx.x][r][1];
         }
@@ -149,66 +149,30 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
 }
 
 // s-interval margin: the float s of SerialBackend.cpp:110 is within ~3e-4 of
-// the exact value for slices up to 2048 px, so a candidate set built from
-// [i-1-kDelta, i+1+kDelta] in exact arithmetic contains every pixel with
-// floor(s_float) in {i-1, i}.
+// the exact value for slices up to 2048 px, so a candidate interval built
+// from [i-1-kDelta, i+1+kDelta] in exact arithmetic holds every pixel with a
+// nonzero factor for bin i.
 constexpr float kDelta = 1.0f / 32.0f;
-// Angles with |cos| >= kRowCos walk rows (LDS kernel): |sin| <= |cos| with
-// chunks of 3 candidates, 0.6 <= |cos| < |sin| with chunks of 4 (a row then
-// holds at most span + 0.02 <= 2.0625/0.6 + 0.02 < 4 of them).
-constexpr float kRowCos = 0.6f;
 
-// Ray walk. A lane's candidates come in chunks of U consecutive pixels of
-// one row, in Serial's raster order: (row y, x..x+U-1).
-//
-// A chunk may reach past the last candidate x1 of its row: such pixels are
-// either inside the image and outside the conservative s-interval (their
-// floor(s) is neither i-1 nor i, so they add v*0), or they are zero pad
-// pixels (x >= w). Either way every partial sum is left unchanged, so no
-// per-candidate mask is needed; chunk starts are clamped to x <= w, which
-// keeps every position within the kPadX pad.
+// Lane geometry of one bin over an image walked by rows, for an angle with
+// |sn| <= |c|. Row y holds the lane's candidates in [xa(y), xa(y) + span],
+// xa(y) = xa0 + y*dxa: the bin's s-interval over the row, stepped from row
+// to row (|dxa| <= 1 and |1/c| <= sqrt2, so the stepping error, ~1e-3 px, is
+// far below the kDelta/|c| and 0.01 px margins), at most span + 0.02 <=
+// 2.0625*sqrt2 + 0.02 < 3 pixels.
 struct RayGeom {
-    float c, sn, cx, cy, dc, slo, shi, inv_c;
-    float xa0, dxa;  // row-stepped interval start, less 0.01 px (|c| >= kRowCos)
-    bool has_c;
+    float c, sn, cx, cy, dc;
+    float xa0, dxa;  // interval start at row 0, less 0.01 px, and its step
     int w;
     // factor of this lane's bin: clamp(min(s*ka + la, s*kb + lb), 0, 1)
     float ka, la, kb, lb;
 };
 
-// First candidate column of row yf for |c| >= kRowCos, as a float in
-// [0, w]: ceil of the stepped interval start clamped to [0, w] (ceil and the
-// clamp commute for an integer w). The row then holds at most 3 candidates
-// when |s| <= |c| (span + 0.02 <= 2.0625*sqrt2 + 0.02 < 3), at most 4 when
-// |c| >= kRowCos. Monotone in the bin: every operation is.
+// First candidate column of row yf, as a float in [0, w]: ceil of the
+// stepped interval start clamped to [0, w] (ceil and the clamp commute for an
+// integer w). Monotone in the bin: every operation is.
 __device__ __forceinline__ float row_first(const RayGeom& g, float yf) {
     return ceilf(__builtin_amdgcn_fmed3f(fmaf(yf, g.dxa, g.xa0), 0.0f, (float)g.w));
-}
-
-// Candidate columns x0..x1 of row y for |c| < kRowCos, and Serial's
-// (y - cy)*sin.
-__device__ __forceinline__ void row_span(const RayGeom& g, int y, int& x0, int& x1, float& sy) {
-    sy = ((float)y - g.cy) * g.sn;
-    x0 = 0;
-    x1 = g.w - 1;
-    if (g.has_c) {
-        const float lo = (g.slo - sy) * g.inv_c;
-        const float hi = (g.shi - sy) * g.inv_c;
-        x0 = max(0, (int)ceilf(fmaxf(g.cx + fminf(lo, hi), -1.0f)));
-        x1 = min(g.w - 1, (int)floorf(fminf(g.cx + fmaxf(lo, hi), (float)g.w)));
-        x0 = min(x0, g.w);
-    }
-}
-
-// Loads of one chunk, the pixels at byte offsets off + k*xstep, k < U.
-template <int U>
-__device__ __forceinline__ void chunk_load(const char* __restrict__ p, unsigned off,
-                                           unsigned xstep, float4 (&v)[U][2]) {
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-        v[k][0] = *reinterpret_cast<const float4*>(p + off + k * xstep);
-        v[k][1] = *reinterpret_cast<const float4*>(p + off + k * xstep + 16);
-    }
 }
 
 // Factor of bin i for a pixel at s. Serial (SerialBackend.cpp:110-120) adds
@@ -268,31 +232,8 @@ __device__ __forceinline__ void chunk_add(const RayGeom& g, float xm0, float sy,
     }
 }
 
-// A lane's walk: rows y0..y1 in order, each as chunks of U pixels from its
-// first candidate. All loads of a chunk are issued before its arithmetic.
-template <int U>
-__device__ __forceinline__ void ray_walk(const RayGeom& g, const char* __restrict__ p,
-                                         unsigned ystep, unsigned xstep, int y0, int y1,
-                                         float (&acc)[kRZ]) {
-    // a chunk starts at x <= x1 <= w - 1 and reaches x + U - 1 <= w + U - 2,
-    // which must stay within the zero pad (pixels w .. w + kPadX - 1)
-    static_assert(U - 2 <= kPadX - 1, "chunk reaches past the zero pad");
-    for (int y = y0; y <= y1; ++y) {
-        int x0, x1;
-        float sy;
-        row_span(g, y, x0, x1, sy);
-        const unsigned roff = (unsigned)y * ystep;
-        for (int x = x0; x <= x1; x += U) {
-            float4 v[U][2];
-            chunk_load<U>(p, roff + (unsigned)x * xstep, xstep, v);
-            chunk_add<U>(g, (float)x - g.cx, sy, v, acc);
-        }
-    }
-}
-
-// Lane geometry of bin i at angle (c, sn); also the candidate rows y0..y1.
-__device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float c, float sn,
-                                           int& y0, int& y1) {
+// Lane geometry of bin i at angle (c, sn), |sn| <= |c|, over a w x h image.
+__device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float c, float sn) {
     RayGeom g;
     g.c = c;
     g.sn = sn;
@@ -301,98 +242,35 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
     g.dc = (float)(bins / 2);
     g.w = w;
     bin_factors(g, i, bins);
-    g.slo = (float)(i - 1) - kDelta - g.dc;  // s - dc lower bound
-    g.shi = (float)(i + 1) + kDelta - g.dc;  // s - dc upper bound
-    const float ac = fabsf(c), as = fabsf(sn);
-
-    // rows that can hold a candidate: (y-cy)*sn in [slo - cx|c|, shi + cx|c|]
-    y0 = 0;
-    y1 = h - 1;
-    if (as > 1e-20f) {
-        const float lo = (g.slo - g.cx * ac) / as;
-        const float hi = (g.shi + g.cx * ac) / as;
-        float ya, yb;
-        if (sn > 0.0f) { ya = g.cy + lo; yb = g.cy + hi; }
-        else           { ya = g.cy - hi; yb = g.cy - lo; }
-        y0 = max(0, (int)floorf(fmaxf(ya, -2.0f)) - 1);
-        y1 = min(h - 1, (int)ceilf(fminf(yb, (float)h + 1.0f)) + 1);
-    }
-
-    // Per-row candidate interval in x: x - cx = ((s-dc) - (y-cy) sn) / c over
-    // s - dc in [slo, shi]. For |c| >= kRowCos it is stepped from row to row,
-    // x in [xa0 + y*dxa, xa0 + y*dxa + span] (|dxa| <= 4/3, |1/c| <= 5/3, so
-    // the stepping error, ~1e-3 px, is far below the kDelta/|c| and 0.01 px
-    // margins).
-    // Steeper rays compute it per row from sy, whose error then scales with
-    // the distance to the strip and vanishes where candidates exist.
-    g.has_c = ac > 1e-20f;
-    g.inv_c = g.has_c ? 1.0f / c : 0.0f;
-    g.xa0 = 0.0f;
-    g.dxa = 0.0f;
-    if (ac >= kRowCos) {
-        const float e0 = (g.slo + g.cy * sn) * g.inv_c, e1 = (g.shi + g.cy * sn) * g.inv_c;
-        g.xa0 = (g.cx + fminf(e0, e1)) - 0.01f;
-        g.dxa = -sn * g.inv_c;
-    }
+    // x - cx = ((s - dc) - (y - cy) sn) / c over s - dc in [slo, shi]
+    const float slo = (float)(i - 1) - kDelta - g.dc;
+    const float shi = (float)(i + 1) + kDelta - g.dc;
+    const float inv_c = 1.0f / c;
+    const float e0 = (slo + g.cy * sn) * inv_c, e1 = (shi + g.cy * sn) * inv_c;
+    g.xa0 = (g.cx + fminf(e0, e1)) - 0.01f;
+    g.dxa = -sn * inv_c;
     return g;
 }
 
-// Rays of angles with |sin| > |cos|: each lane walks its rows with direct
-// loads. Lanes whose ray enters through the top edge (first candidate row
-// 0) are on one common row at every step, spaced 1/|cos| pixels apart: they
-// read the row-major copy. The others enter through a side edge and are on
-// consecutive rows of one common column at every step: they read the
-// column-major copy. Either way a wave's loads are close to contiguous.
-//
-// A launch is one round of the XCD-grouped order (xcd_map): kXcds / gsz
-// z-blocks from zb0 (nzl of them present), each walked by one group of gsz
-// XCDs; a z-block's slots are its (bin block, angle) pairs.
-template <int U>
-__global__ __launch_bounds__(256) void radon_ray_kernel(
-    const float4* __restrict__ rowL, const float4* __restrict__ colL, int w, int h, int nz,
-    int na, int bins, const float* __restrict__ cos_t, const float* __restrict__ sin_t,
-    const int* __restrict__ alist, float* __restrict__ sino, int zb0, int nzl, int gsz,
-    int n_dir) {
-    const int bb = (bins + 255) / 256;
-    const int slots = (bb * n_dir + gsz - 1) / gsz;
-    int zl, slot;
-    if (!xcd_map(blockIdx.x, gsz, slots, bb * n_dir, nzl, zl, slot)) return;
-    const int i = (slot % bb) * 256 + threadIdx.x;  // bin owned by this lane
-    const int a = alist[slot / bb];
-    const int zb = zb0 + zl;
-    if (i >= bins) return;
-    int y0, y1;
-    const RayGeom g = ray_geom(i, bins, w, h, cos_t[a], sin_t[a], y0, y1);
-    // one z-block's copy; byte offsets within it fit 32 bits
-    const unsigned wp = (unsigned)(w + kPadX);
-    const bool top = (y0 == 0);
-    const char* p = reinterpret_cast<const char*>(top ? rowL : colL) + (size_t)zb * radon_zbytes(w, h);
-    // row copy: x steps 32 bytes, y steps 32 (w + kPadX);
-    // column copy: x steps 32 h bytes, y steps 32
-    const unsigned ystep = top ? 32u * wp : 32u;
-    const unsigned xstep = top ? 32u : 32u * (unsigned)h;
-    float acc[kRZ];
-#pragma unroll
-    for (int k = 0; k < kRZ; ++k) acc[k] = 0.0f;
-    ray_walk<U>(g, p, ystep, xstep, y0, y1, acc);
-#pragma unroll
-    for (int k = 0; k < kRZ; ++k) {
-        const int z = zb * kRZ + k;
-        if (z < nz) sino[((size_t)z * na + a) * bins + i] = acc[k];
-    }
-}
-
-// Rays of angles with |cos| >= kRowCos: every lane of a block (256 bins of
-// one angle and z-block) walks all rows top to bottom, at most U candidates
-// per row. The block stages RR rows at a time into LDS, each as a window of kRowW
-// pixels starting at the leftmost lane's first candidate. A lane's first
+// Row walk over a w x h image copy [zb][h][w + kPadX] for angles with
+// |sin| <= |cos|: every lane of a block (256 bins of one angle and z-block)
+// walks all rows top to bottom with one chunk of U = 3 candidates per row.
+// The block stages RR rows at a time into LDS, each as a window of kRowW
+// pixels starting at the extreme lane's first candidate. A lane's first
 // candidate x0(y) is a composition of monotone float operations of its bin,
 // so the window start is that of the extreme bin (the lowest one for
 // cos > 0, the highest for cos < 0), and every lane's chunk lies within
-// kRowW pixels of it: lanes 255 bins apart are at most 255/|cos| pixels
-// apart (360.7 for |sin| <= |cos|, 425 for |cos| >= 0.6), plus a chunk of
-// 3 or 4 pixels. Rows outside a lane's candidate rows hold no candidates of
-// its bin and add +0.
+// kRowW pixels of it: lanes 255 bins apart are at most 255/|cos| <= 360.7
+// pixels apart, plus a chunk of 3. Rows outside a lane's candidate rows add
+// +0.
+//
+// The other angles walk the transposed image: the column copy
+// [zb][w][h + kPadX] is the row copy of the transpose, and the kernel runs on
+// it with (w, h) and (cos, sin) exchanged. Serial's s = (x-cx)*cos +
+// (y-cy)*sin + dc is then formed as (y-cy)*sin + (x-cx)*cos + dc, the same
+// float value (the first addition commutes exactly), so every pixel's bin and
+// fraction are Serial's; each bin sums its deposits column by column instead
+// of row by row.
 constexpr int kRadonRows = 2;  // rows staged per step
 
 // Workgroups follow the XCD-grouped order (xcd_map): units are z-blocks,
@@ -405,6 +283,7 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
     // window: lanes 255 bins apart are at most 255/|cos| pixels apart, plus a
     // chunk of U; the window must also keep every chunk inside the zero pad
     static_assert(U - 2 <= kPadX - 1, "chunk reaches past the zero pad");
+    static_assert(255.0 * 1.41421357 + U <= kRowW, "window narrower than the lanes' span");
     __shared__ float4 t0[RR][kRowW];  // slices 0-3 of the staged pixels
     __shared__ float4 t1[RR][kRowW];  // slices 4-7
     __shared__ int xw_s[2][RR];       // window starts: staged rows, next rows
@@ -418,8 +297,7 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
     const int i = b0 + threadIdx.x;
     const bool live = i < bins;
     const float c = cos_t[a], sn = sin_t[a];
-    int y0, y1;
-    RayGeom g = ray_geom(live ? i : bins - 1, bins, w, h, c, sn, y0, y1);
+    RayGeom g = ray_geom(live ? i : bins - 1, bins, w, h, c, sn);
     if (!live) {  // walks along, adds nothing: factor clamp(-1) = 0
         g.ka = g.kb = 0.0f;
         g.la = g.lb = -1.0f;
@@ -432,8 +310,7 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
     const int ie = c > 0.0f ? b0 : min(b0 + 255, bins - 1);
     RayGeom ge;
     if (threadIdx.x < RR) {
-        int ey0, ey1;
-        ge = ray_geom(ie, bins, w, h, c, sn, ey0, ey1);
+        ge = ray_geom(ie, bins, w, h, c, sn);
         xw_s[0][threadIdx.x] = (int)row_first(ge, (float)threadIdx.x);
     }
     // Staging: waves 0-1 stage the step's first row, waves 2-3 its second;
@@ -524,69 +401,24 @@ hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hip
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, hipStream_t s) {
     const int nzb = (nz + kRZ - 1) / kRZ;
-    dim3 grid((w + kPadX + 31) / 32, (h + 31) / 32, nzb);
+    dim3 grid((w + kPadX + 31) / 32, (h + kPadX + 31) / 32, nzb);
     hipLaunchKernelGGL(normalise_kernel, grid, dim3(32, 8), 0, s, d_vol, w, h, nz, d_mm, d_row,
                        d_col);
     return hipGetLastError();
 }
 
-hipError_t launch_radon_rows(const float4* d_row, int w, int h, int nz, int na, int bins,
-                             const float* d_cos, const float* d_sin, const int* d_alist,
-                             int n_row3, int n_row4, float* d_sino, hipStream_t s) {
+hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, int bins,
+                             const float* d_cos, const float* d_sin, const int* d_alist, int n,
+                             float* d_sino, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int bb = (bins + 255) / 256;
-    // workgroups of the XCD-grouped order for n angles
-    auto grid = [&](int n, int& gsz) {
-        gsz = xcd_group(bb * n, nzb);
-        const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
-        return dim3((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
-    };
-    int gsz = 1;
-    if (n_row3 > 0) {  // |sin| <= |cos|: 3 candidates, window 368
-        const dim3 g = grid(n_row3, gsz);
-        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 368>), g, dim3(256), 0, s, d_row, w,
-                           h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n_row3, gsz);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    if (n_row4 > 0) {  // 0.6 <= |cos| < |sin|: 4 candidates, window 432
-        const dim3 g = grid(n_row4, gsz);
-        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 4, 432>), g, dim3(256), 0, s, d_row, w,
-                           h, nz, na, bins, d_cos, d_sin, d_alist + n_row3, d_sino, n_row4, gsz);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
-}
-
-hipError_t launch_radon_direct(const float4* d_row, const float4* d_col, int w, int h, int nz,
-                               int na, int bins, const float* d_cos, const float* d_sin,
-                               const int* d_alist, int n_dir, float* d_sino, hipStream_t s,
-                               int* launches) {
-    if (launches) *launches = 0;
-    if (n_dir <= 0) return hipSuccess;
-    const int nzb = (nz + kRZ - 1) / kRZ;
-    const int bb = (bins + 255) / 256;
-    // XCDs per z-block: about 300 blocks of the z-block per XCD and launch,
-    // about 1.5 times what an XCD holds at once (two XCDs per z-block at 512
-    // bins x 295 angles, four at 1024 bins: the best of 1, 2, 4 and 8 at
-    // both). The blocks of one launch start together and sweep their
-    // z-block's copies in step, so most pixels come from HBM once per launch.
-    const int zbb = bb * n_dir;
-    const int gsz = xcd_group(zbb, nzb);
-    const int per = kXcds / gsz;
-    const int slots = (zbb + gsz - 1) / gsz;  // workgroups per XCD
-    for (int zb0 = 0; zb0 < nzb; zb0 += per) {
-        // chunks of 4 candidates (8 loads in flight per lane); smaller
-        // chunks load fewer pixels past the runs but measured slower
-        hipLaunchKernelGGL(radon_ray_kernel<4>, dim3(kXcds * slots), dim3(256), 0, s, d_row,
-                           d_col, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, zb0,
-                           std::min(per, nzb - zb0), gsz, n_dir);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        if (launches) ++*launches;
-    }
-    return hipSuccess;
+    const int gsz = xcd_group(bb * n, nzb);
+    const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
+    const dim3 grid((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
+    hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 368>), grid, dim3(256), 0, s, d_img, w,
+                       h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n, gsz);
+    return hipGetLastError();
 }
 
 }  // namespace cthip
