@@ -149,9 +149,10 @@ int ct_hip_extract_point_cloud_dev(ct_hip_ctx* ctx, const float* d_vol, int w, i
                                    float threshold, float* out_xyzh, size_t capacity,
                                    size_t* count);
 
-/* Replaces FilterDesign::createFilter (FilterDesign.h:11; FilterDesign.cpp:13-111),
- * designed in FP64 on the device: n/2+1 taps, DC and Nyquist zeroed (the
- * effective Serial filter: FFT::inverseReal never reads the Nyquist bin). */
+/* Replaces FilterDesign::createFilter (FilterDesign.h:11; FilterDesign.cpp:13-111):
+ * its n/2+1 taps bit for bit (the reference's FP32 FFT of the spatial ramp,
+ * windowed in double; computed on the host). The row filter uses these taps
+ * with the Nyquist bin zeroed, as FFT::inverseReal never reads it. */
 int ct_hip_design_filter(ct_hip_ctx* ctx, int n, int filter, float* out);
 
 /* Row-filter backend of the context, the analogue of the reference's

@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract
           "-I", os.path.join(os.path.dirname(PKG), "include")]
 
 SOURCES = ["phantom.hip", "radon.hip", "filter.hip", "backproject.hip", "pointcloud.hip",
-           "filter_gemm.hip", "capi.cpp", "batch.cpp", "serve.cpp"]
+           "filter_gemm.hip", "filter_design.cpp", "capi.cpp", "batch.cpp", "serve.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -115,13 +115,22 @@ std::vector<float> serial_angles(int na) {
 
 int norm_filter(int f) { return (f >= 0 && f <= 5) ? f : CT_FILTER_SHEPP_LOGAN; }
 
+// Taps of the FFT row filter: the reference's createFilter taps with the
+// Nyquist bin set to 0, which the filter then drops as Serial's inverse real
+// FFT does (FFT.cpp:87-120 never reads bin n/2).
 float* get_filter(ct_hip_ctx* c, int nfft, int type) {
     type = norm_filter(type);
     auto k = std::make_pair(nfft, type);
     auto it = c->filters.find(k);
     if (it != c->filters.end()) return it->second;
-    float* H = static_cast<float*>(dmalloc(c, sizeof(float) * (nfft / 2 + 1)));
-    hip_check(launch_filter_design(nfft, type, H, c->stream), "filter design");
+    std::vector<float> h(nfft / 2 + 1);
+    serial_filter_taps(nfft, type, h.data());
+    h[nfft / 2] = 0.0f;
+    float* H = static_cast<float*>(dmalloc(c, sizeof(float) * h.size()));
+    hip_check(hipMemcpyAsync(H, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice,
+                             c->stream),
+              "upload filter taps");
+    hip_check(hipStreamSynchronize(c->stream), "filter taps");
     c->filters[k] = H;
     return H;
 }
@@ -745,11 +754,7 @@ int ct_hip_design_filter(ct_hip_ctx* c, int n, int filter, float* out) {
     return guarded(c, [&] {
         need(out != nullptr, "designFilter: null output");
         need(n >= 4 && (n & (n - 1)) == 0, "designFilter: n must be a power of two >= 4");
-        const float* H = get_filter(c, n, filter);
-        hip_check(hipMemcpyAsync(out, H, sizeof(float) * (n / 2 + 1), hipMemcpyDeviceToHost,
-                                 c->stream),
-                  "download filter");
-        hip_check(hipStreamSynchronize(c->stream), "sync");
+        serial_filter_taps(n, norm_filter(filter), out);
     });
 }
 

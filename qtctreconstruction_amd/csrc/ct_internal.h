@@ -133,7 +133,9 @@ int filter_persistent_grid(int max_nfft, int device);
 // at launch. Everything batch-dependent is read from device memory.
 hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
                                         int max_nfft, int grid, hipStream_t s);
-hipError_t launch_filter_design(int n, int type, float* d_H, hipStream_t s);
+// FilterDesign::createFilter(n, type): n/2 + 1 taps, the reference's FP32
+// arithmetic bit for bit (host; n a power of two >= 4)
+void serial_filter_taps(int n, int type, float* H);
 hipError_t launch_twiddles(int n, float2* d_tw, hipStream_t s);
 // sino [nz][na][bins] -> filt [nzb][na][sq] float4 (4 slices interleaved)
 hipError_t launch_row_filter_fft(const float* d_sino, int nz, int na, int bins, int sq, int pad,

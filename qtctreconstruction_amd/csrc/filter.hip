@@ -1,5 +1,5 @@
-// filter.hip — row filter of the FBP: filter design (FP64) and the padded
-// FFT convolution of every sinogram row.
+// filter.hip — row filter of the FBP: the padded FFT convolution of every
+// sinogram row, and the twiddle table.
 //
 // Reference semantics (SerialBackend.cpp:150-195, FilterDesign.cpp:13-111,
 // FFT.cpp:51-120): each row of `bins` samples is placed at offset
@@ -7,11 +7,8 @@
 // transformed, multiplied by the real filter H (n/2+1 taps, H[0] = 0),
 // transformed back, and the first sq samples are kept. The reference's
 // inverse never reads the Nyquist bin, so the effective filter also has
-// H[n/2] = 0; the design here zeroes it.
-//
-// Design: FP64 on the device, by direct summation of the DFT of the
-// reference's float kernel f (f[0]=0.25, f[odd m] = -1/(pi m)^2 rounded to
-// float), times the window, stored as float.
+// H[n/2] = 0. H holds the reference's own taps (FP32 design, computed on the
+// host once per (n, type)) with that bin zeroed.
 //
 // Filtering: the filter is real and even, so two real rows ride in one
 // complex FFT (row A in the real part, row B in the imaginary part) and come
@@ -28,40 +25,6 @@
 #include "ct_internal.h"
 
 namespace cthip {
-
-__global__ void filter_design_kernel(int n, int type, float* __restrict__ H) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nh = n / 2;
-    if (k > nh) return;
-    const double pi = 3.14159265358979323846;
-    double acc = 0.25;  // f[0] * cos(0)
-    for (int j = 1; j < n; j += 2) {
-        const int m = (j < nh) ? j : n - j;
-        const double t = 1.0 / (pi * (double)m);
-        const double fj = (double)(float)(-1.0 * t * t);
-        const long long r = ((long long)j * k) % n;
-        acc += fj * cos(2.0 * pi * (double)r / (double)n);
-    }
-    double h = 2.0 * acc;
-    const double freq = (double)k / (double)n;
-    switch (type) {
-        case 0: break;                                                  // Ramp
-        case 2: h *= 0.54 + 0.46 * cos(pi * freq / 0.5); break;        // Hamming
-        case 3: h *= cos(pi * freq); break;                             // Cosine
-        case 4: h *= 0.5 + 0.5 * cos(pi * freq / 0.5); break;          // Hann
-        case 5: {                                                        // Bartlett
-            const double r = fabs(freq) / 0.5;
-            h *= 1.0 - (r < 1.0 ? r : 1.0);
-        } break;
-        case 1:
-        default: {  // Shepp-Logan (also every unknown id)
-            const double om = pi * freq;
-            if (k > 0) h *= sin(om) / om;
-        } break;
-    }
-    if (k == 0 || k == nh) h = 0.0;
-    H[k] = (float)h;
-}
 
 // Twiddles in stage order: the table of a butterfly span m (m = 2, 4, ..., n)
 // starts at m/2 - 1 and holds W_m^j = exp(-2 pi i j / m) for j < m/2, so a
@@ -412,13 +375,6 @@ hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr,
     }
     hipLaunchKernelGGL(row_filter_persistent_kernel, dim3(grid), dim3(256), fft_lds_bytes(max_nfft),
                        s, d_jobs, d_hdr, d_counter);
-    return hipGetLastError();
-}
-
-hipError_t launch_filter_design(int n, int type, float* d_H, hipStream_t s) {
-    const int cnt = n / 2 + 1;
-    hipLaunchKernelGGL(filter_design_kernel, dim3((cnt + 127) / 128), dim3(128), 0, s, n, type,
-                       d_H);
     return hipGetLastError();
 }
 

@@ -147,12 +147,17 @@ def test_fft_filter_every_length(backend, O, bins):
         assert O.rel_l2(got, ref) <= TOL, (bins, filt)
 
 
-def test_filter_design_on_device(backend, O):
-    for n in (64, 512, 2048, 4096):
-        for t in range(6):
-            got = backend.ctx.design_filter(n, t).astype(np.float64)
-            ref = O.filter64(n, t)
-            assert np.max(np.abs(got - ref)) <= 1e-6 * max(1.0, np.abs(ref).max())
+def test_filter_design_bit_exact(backend, O):
+    """designFilter returns FilterDesign::createFilter's taps bit for bit
+    (unknown types fall back to Shepp-Logan), within 2.5e-7 of the FP64
+    design."""
+    for n in (4, 64, 512, 2048, 4096):
+        for t in (0, 1, 2, 3, 4, 5, 9):
+            got = backend.ctx.design_filter(n, t)
+            assert _bits_equal(got, O.create_filter(n, t)), (n, t)
+            if n >= 64 and t <= 5:
+                ref = O.filter64(n, t)[: n // 2]  # the FP64 design zeroes bin n/2
+                assert np.max(np.abs(got[: n // 2] - ref)) <= 3e-7, (n, t)
 
 
 @pytest.mark.parametrize("out_size,angles", [(64, "serial"), (100, "serial"), (40, "random")])
