@@ -38,8 +38,8 @@ FLOP_PER_SAMPLE = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows_kernel": 2.067e+10,
-    "bp_kernel": 6.534e+09,
+    "radon_rows_kernel": 8.735e+09,
+    "bp_kernel": 6.500e+09,
 }
 
 
