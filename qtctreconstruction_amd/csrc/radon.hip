@@ -274,26 +274,33 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
 constexpr int kRadonRows = 2;  // rows staged per step
 
 // Workgroups follow the XCD-grouped order (xcd_map): units are z-blocks,
-// a z-block's slots its (bin block, angle) pairs, n_ang angles.
-template <int RR, int U, int kRowW>
-__global__ __launch_bounds__(256) void radon_rows_kernel(
+// a z-block's slots its (bin block, angle) pairs, n_ang angles. A block has
+// NT lanes (NT bins). FULL: the image row fits the window (w + kPadX <=
+// kRowW), every block stages whole rows and the window start is 0; NT = 512
+// then, 8 waves sharing 2 staged rows (32 waves per CU at 516-pixel rows,
+// against 24 for the windowed 256-lane form, whose LDS per wave is larger).
+template <int RR, int U, int kRowW, int NT, bool FULL>
+__global__ __launch_bounds__(NT) void radon_rows_kernel(
     const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
     const float* __restrict__ cos_t, const float* __restrict__ sin_t,
     const int* __restrict__ alist, float* __restrict__ sino, int n_ang, int gsz) {
-    // window: lanes 255 bins apart are at most 255/|cos| pixels apart, plus a
-    // chunk of U; the window must also keep every chunk inside the zero pad
+    // window: lanes NT-1 bins apart are at most (NT-1)/|cos| pixels apart,
+    // plus a chunk of U; the window must also keep every chunk inside the
+    // zero pad
     static_assert(U - 2 <= kPadX - 1, "chunk reaches past the zero pad");
-    static_assert(255.0 * 1.41421357 + U <= kRowW, "window narrower than the lanes' span");
+    static_assert(FULL || (NT - 1) * 1.41421357 + U <= kRowW,
+                  "window narrower than the lanes' span");
+    static_assert(RR == 2 && NT % 128 == 0, "two staged rows, one per half of the block");
     __shared__ float4 t0[RR][kRowW];  // slices 0-3 of the staged pixels
     __shared__ float4 t1[RR][kRowW];  // slices 4-7
     __shared__ int xw_s[2][RR];       // window starts: staged rows, next rows
-    const int bb = (bins + 255) / 256;
+    const int bb = (bins + NT - 1) / NT;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int slots = (bb * n_ang + gsz - 1) / gsz;
     int zb, slot;
     if (!xcd_map(blockIdx.x, gsz, slots, bb * n_ang, nzb, zb, slot)) return;  // whole block
     const int a = alist[slot / bb];
-    const int b0 = (slot % bb) * 256;
+    const int b0 = (slot % bb) * NT;
     const int i = b0 + threadIdx.x;
     const bool live = i < bins;
     const float c = cos_t[a], sn = sin_t[a];
@@ -307,19 +314,25 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
         reinterpret_cast<const char*>(rowL) + (size_t)zb * radon_zbytes(w, h);
     // the extreme bin's geometry, held by the first RR lanes only: lane r
     // computes the window start of row ys + r of the next step
-    const int ie = c > 0.0f ? b0 : min(b0 + 255, bins - 1);
+    const int ie = c > 0.0f ? b0 : min(b0 + NT - 1, bins - 1);
     RayGeom ge;
     if (threadIdx.x < RR) {
-        ge = ray_geom(ie, bins, w, h, c, sn);
-        xw_s[0][threadIdx.x] = (int)row_first(ge, (float)threadIdx.x);
+        if (FULL) {
+            xw_s[0][threadIdx.x] = 0;
+        } else {
+            ge = ray_geom(ie, bins, w, h, c, sn);
+            xw_s[0][threadIdx.x] = (int)row_first(ge, (float)threadIdx.x);
+        }
     }
-    // Staging: waves 0-1 stage the step's first row, waves 2-3 its second;
-    // thread column j = (threadIdx.x & 127) + 128 k, k < NE. Columns past
-    // kRowW load a clamped in-bounds address and are not written.
-    static_assert(RR == 2, "two staged rows, one per wave pair");
-    constexpr int NE = (kRowW + 127) / 128;
-    const int srow = __builtin_amdgcn_readfirstlane(threadIdx.x >> 7);
-    const int sj0 = threadIdx.x & 127;
+    // Staging: the first half of the block stages the step's first row, the
+    // second half its second; thread column j = (threadIdx.x % (NT/2)) +
+    // (NT/2) k, k < NE. Columns past the window (FULL: past the row) load a
+    // clamped in-bounds address and are not written.
+    constexpr int HB = NT / 2;
+    constexpr int NE = (kRowW + HB - 1) / HB;
+    const int srow = __builtin_amdgcn_readfirstlane(threadIdx.x / HB);
+    const int sj0 = threadIdx.x % HB;
+    const int nstage = FULL ? wp : kRowW;
 
     float acc[kRZ];
 #pragma unroll
@@ -327,30 +340,30 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
     int buf = 0;
     for (int ys = 0; ys < h; ys += RR, buf ^= 1) {
         __syncthreads();  // the previous rows are consumed; xw_s[buf] is set
-        if (threadIdx.x < RR)  // window starts of the next step's rows
+        if (!FULL && threadIdx.x < RR)  // window starts of the next step's rows
             xw_s[buf ^ 1][threadIdx.x] = (int)row_first(ge, (float)(ys + RR + threadIdx.x));
         // Lanes read window slots up to x0 + U - 1 <= w + kPadX - 1, inside
         // the zero pad, and rows below h only: slots past the pad or the last
         // row are never read, so their addresses are clamped instead of
         // tested. All loads are issued before the LDS writes.
         {
-            // this wave pair's row (clamped to the last row) and window
-            // start, both wave-uniform
+            // this half's row (clamped to the last row) and window start,
+            // both wave-uniform
             const unsigned ro = (unsigned)min(ys + srow, h - 1) * (unsigned)wp * 32u;
-            const int xr = __builtin_amdgcn_readfirstlane(xw_s[buf][srow]);
+            const int xr = FULL ? 0 : __builtin_amdgcn_readfirstlane(xw_s[buf][srow]);
             float4 q0[NE], q1[NE];
 #pragma unroll
             for (int k = 0; k < NE; ++k) {
-                const int x = min(xr + sj0 + 128 * k, w + kPadX - 1);
+                const int x = min(xr + sj0 + HB * k, w + kPadX - 1);
                 const unsigned off = ro + ((unsigned)x << 5);
                 q0[k] = *reinterpret_cast<const float4*>(slab + off);
                 q1[k] = *reinterpret_cast<const float4*>(slab + off + 16);
             }
 #pragma unroll
             for (int k = 0; k < NE; ++k) {
-                if (sj0 + 128 * k < kRowW) {
-                    t0[srow][sj0 + 128 * k] = q0[k];
-                    t1[srow][sj0 + 128 * k] = q1[k];
+                if (sj0 + HB * k < nstage) {
+                    t0[srow][sj0 + HB * k] = q0[k];
+                    t1[srow][sj0 + HB * k] = q1[k];
                 }
             }
         }
@@ -359,7 +372,7 @@ __global__ __launch_bounds__(256) void radon_rows_kernel(
             const float yf = (float)(ys + r);
             const float sy = (yf - g.cy) * g.sn;  // SerialBackend.cpp:110
             const float x0f = row_first(g, yf);
-            const int d = (int)x0f - xw_s[buf][r];
+            const int d = FULL ? (int)x0f : (int)x0f - xw_s[buf][r];
             float4 v[U][2];
 #pragma unroll
             for (int k = 0; k < U; ++k) {
@@ -412,12 +425,18 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
                              float* d_sino, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int nzb = (nz + kRZ - 1) / kRZ;
-    const int bb = (bins + 255) / 256;
+    const bool full = w + kPadX <= 520;  // whole rows fit the 512-lane form's window
+    const int NT = full ? 512 : 256;
+    const int bb = (bins + NT - 1) / NT;
     const int gsz = xcd_group(bb * n, nzb);
     const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
     const dim3 grid((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
-    hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 368>), grid, dim3(256), 0, s, d_img, w,
-                       h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n, gsz);
+    if (full)
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 520, 512, true>), grid, dim3(512), 0,
+                           s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n, gsz);
+    else
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 368, 256, false>), grid, dim3(256),
+                           0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n, gsz);
     return hipGetLastError();
 }
 

@@ -65,7 +65,8 @@ def _slice(rng, h, w, zero_frac=0.3):
 
 @pytest.mark.parametrize("h,w,na,bins", [(64, 64, 180, 64), (97, 97, 33, 97), (50, 64, 45, 70),
                                          (64, 50, 7, 40), (32, 32, 1, 32), (128, 128, 360, 128),
-                                         (20, 20, 12, 3), (9, 8, 16, 1), (12, 10, 30, 2)])
+                                         (20, 20, 12, 3), (9, 8, 16, 1), (12, 10, 30, 2),
+                                         (24, 600, 20, 600), (600, 24, 20, 300)])
 def test_sinogram_parity(backend, O, h, w, na, bins):
     rng = np.random.default_rng(h * 1000 + na)
     s = _slice(rng, h, w)
@@ -89,7 +90,8 @@ def test_sinogram_phantom_slice_and_constant(backend, O):
 
 
 @pytest.mark.parametrize("h,w,na,bins", [(97, 97, 180, 97), (64, 50, 90, 70), (50, 64, 45, 40),
-                                         (20, 20, 12, 3), (33, 33, 360, 33)])
+                                         (20, 20, 12, 3), (33, 33, 360, 33), (16, 700, 24, 700),
+                                         (700, 16, 24, 100)])
 def test_sinogram_indexing_bit_exact(backend, O, h, w, na, bins):
     """Every pixel's bins and fraction are Serial's at every angle: with one
     nonzero pixel in the slice each bin receives at most one deposit, so the
