@@ -58,6 +58,14 @@ typedef const float __attribute__((address_space(1))) g_cfloat;
 typedef const f2v __attribute__((address_space(1))) g_cfloat2;
 typedef f4v __attribute__((address_space(1))) g_float4;
 
+// LDS slot of transform element e: bits 0-4 of e are XORed with bits 3-7, a
+// permutation within each 32-element (256-byte) block. The butterfly passes
+// read elements base + r*h; for small spans h, the lanes of one b64 read
+// would otherwise land on a few of the 64 banks (up to 8-way conflicts at
+// h = 1, 4); swizzled, the mean conflict degree over every pass of n = 64 ..
+// 4096 is 1.07 (from 4.1).
+__device__ __forceinline__ int swz(int e) { return e ^ ((e >> 3) & 31); }
+
 // Radix-2 butterfly stages on the two LDS transforms X0, X1 of n points.
 // Stage of half-span m, element e at position q = e mod 2m < m paired with
 // e + m; W = W_2m^q (stage order: W_M^j at M/2 - 1 + j), conjugated when
@@ -90,7 +98,7 @@ __device__ __forceinline__ void fft_dit8(float2* X0, float2* X1, int n, int h, c
             float2* X = f ? X1 : X0;
             float2 x[8];
 #pragma unroll
-            for (int r = 0; r < 8; ++r) x[r] = X[base + r * h];
+            for (int r = 0; r < 8; ++r) x[r] = X[swz(base + r * h)];
 #pragma unroll
             for (int r = 0; r < 8; r += 2) {  // half-span h
                 const float2 t = twmul<CONJ>(x[r + 1], w1);
@@ -113,7 +121,7 @@ __device__ __forceinline__ void fft_dit8(float2* X0, float2* X1, int n, int h, c
                 x[r] = make_float2(x[r].x + t.x, x[r].y + t.y);
             }
 #pragma unroll
-            for (int r = 0; r < 8; ++r) X[base + r * h] = x[r];
+            for (int r = 0; r < 8; ++r) X[swz(base + r * h)] = x[r];
         }
     }
     __syncthreads();
@@ -135,7 +143,7 @@ __device__ __forceinline__ void fft_dif8(float2* X0, float2* X1, int n, int h, c
             float2* X = f ? X1 : X0;
             float2 x[8];
 #pragma unroll
-            for (int r = 0; r < 8; ++r) x[r] = X[base + r * h];
+            for (int r = 0; r < 8; ++r) x[r] = X[swz(base + r * h)];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {  // half-span 4h
                 const float2 d = make_float2(x[r].x - x[r + 4].x, x[r].y - x[r + 4].y);
@@ -158,7 +166,7 @@ __device__ __forceinline__ void fft_dif8(float2* X0, float2* X1, int n, int h, c
                 x[r + 1] = twmul<CONJ>(d, w1);
             }
 #pragma unroll
-            for (int r = 0; r < 8; ++r) X[base + r * h] = x[r];
+            for (int r = 0; r < 8; ++r) X[swz(base + r * h)] = x[r];
         }
     }
     __syncthreads();
@@ -176,7 +184,7 @@ __device__ __forceinline__ void fft_dit4(float2* X0, float2* X1, int n, int h, c
             float2* X = f ? X1 : X0;
             float2 x[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) x[r] = X[base + r * h];
+            for (int r = 0; r < 4; ++r) x[r] = X[swz(base + r * h)];
 #pragma unroll
             for (int r = 0; r < 4; r += 2) {
                 const float2 t = twmul<CONJ>(x[r + 1], w1);
@@ -190,7 +198,7 @@ __device__ __forceinline__ void fft_dit4(float2* X0, float2* X1, int n, int h, c
             x[3] = make_float2(x[1].x - t1.x, x[1].y - t1.y);
             x[1] = make_float2(x[1].x + t1.x, x[1].y + t1.y);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) X[base + r * h] = x[r];
+            for (int r = 0; r < 4; ++r) X[swz(base + r * h)] = x[r];
         }
     }
     __syncthreads();
@@ -208,7 +216,7 @@ __device__ __forceinline__ void fft_dif4(float2* X0, float2* X1, int n, int h, c
             float2* X = f ? X1 : X0;
             float2 x[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) x[r] = X[base + r * h];
+            for (int r = 0; r < 4; ++r) x[r] = X[swz(base + r * h)];
             const float2 d0 = make_float2(x[0].x - x[2].x, x[0].y - x[2].y);
             x[0] = make_float2(x[0].x + x[2].x, x[0].y + x[2].y);
             x[2] = twmul<CONJ>(d0, w2a);
@@ -222,7 +230,7 @@ __device__ __forceinline__ void fft_dif4(float2* X0, float2* X1, int n, int h, c
                 x[r + 1] = twmul<CONJ>(d, w1);
             }
 #pragma unroll
-            for (int r = 0; r < 4; ++r) X[base + r * h] = x[r];
+            for (int r = 0; r < 4; ++r) X[swz(base + r * h)] = x[r];
         }
     }
     __syncthreads();
@@ -231,7 +239,7 @@ __device__ __forceinline__ void fft_dif4(float2* X0, float2* X1, int n, int h, c
 // half-span 1 (twiddle 1)
 __device__ __forceinline__ void fft_dit2(float2* X0, float2* X1, int n, int tid) {
     for (int b = tid; b < n / 2; b += 256) {
-        const int i0 = 2 * b, i1 = i0 + 1;
+        const int i0 = swz(2 * b), i1 = swz(2 * b + 1);
         const float2 u0 = X0[i0], v0 = X0[i1], u1 = X1[i0], v1 = X1[i1];
         X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
         X0[i1] = make_float2(u0.x - v0.x, u0.y - v0.y);
@@ -270,8 +278,8 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
 #pragma unroll
         for (int r = 0; r < kZB; ++r)
             v[r] = (rows[r] && jj >= 0 && jj < p.bins) ? rows[r][jj] : 0.0f;
-        X0[j] = make_float2(v[0], v[1]);
-        X1[j] = make_float2(v[2], v[3]);
+        X0[swz(j)] = make_float2(v[0], v[1]);
+        X1[swz(j)] = make_float2(v[2], v[3]);
     }
     __syncthreads();
     // Forward: decimation in frequency from natural-order input to
@@ -293,8 +301,9 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     for (int q = tid; q < n; q += 256) {
         const int k = (int)(__brev((unsigned)q) >> (32 - logn));
         const float hv = H[k <= nh ? k : n - k];
-        X0[q] = make_float2(X0[q].x * hv, X0[q].y * hv);
-        X1[q] = make_float2(X1[q].x * hv, X1[q].y * hv);
+        const int qs = swz(q);
+        X0[qs] = make_float2(X0[qs].x * hv, X0[qs].y * hv);
+        X1[qs] = make_float2(X1[qs].x * hv, X1[qs].y * hv);
     }
     __syncthreads();
     h = 1;
@@ -309,7 +318,7 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     const float inv_n = 1.0f / (float)n;
     g_float4* dst = (g_float4*)p.filt + ((size_t)zb * p.na + a) * p.sq;
     for (int i = tid; i < p.sq; i += 256) {
-        const float2 u = X0[i], v = X1[i];
+        const float2 u = X0[swz(i)], v = X1[swz(i)];
         const f4v o = {u.x * inv_n, u.y * inv_n, v.x * inv_n, v.y * inv_n};
         dst[i] = o;
     }
