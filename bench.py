@@ -38,8 +38,8 @@ FLOP_PER_SAMPLE = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows_kernel": 8.735e+09,
-    "bp_kernel": 6.500e+09,
+    "radon_rows_kernel": 1.0643e+10,
+    "bp_kernel": 6.524e+09,
 }
 
 
