@@ -1,8 +1,9 @@
 """Benchmark of the FBP hot path on one MI355X.
 
 A step is one reconstructVolume over device-resident input: per-slice
-min/max + normalisation, ray-driven Radon (720 angles x 512 bins; an
-LDS-staged row walk and a direct walk, each timed with HIP events), the
+min/max + normalisation, ray-driven Radon (720 angles x 512 bins; the
+LDS-staged row walk over the image and over its transpose, each timed with
+HIP events), the
 Shepp-Logan FFT row filter (sq 725, nfft 2048) and voxel-driven
 backprojection with denormalisation, for a 512^3 phantom of 24 seeded
 ellipsoids (the plain-FP32 single-GPU configuration). value = voxel-updates/s

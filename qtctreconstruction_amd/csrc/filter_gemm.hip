@@ -47,7 +47,8 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
 
-// ---- design: H in FP64 (same formula as filter_design_kernel), then h ----
+// ---- design: H in FP64 (the exact DFT of the reference's float spatial ramp
+// taps, windowed), then h ----
 __global__ void filter_design64_kernel(int n, int type, double* __restrict__ H) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int nh = n / 2;
