@@ -139,6 +139,30 @@ def precision_form(ctx, d_sino, n, na, sample_z, host_sinos_sample):
     return res
 
 
+def metrics_form(ctx, d_vol, d_out, nvox, vol_host, reps=10):
+    """The controller's reconstruction metrics (MSE/MAE/max/PSNR) of the
+    reconstruction against its phantom, both resident on the device: one
+    HBM-bound pass over 8 bytes per voxel. Wall time per call includes the
+    32-byte result download and the stream synchronisation."""
+    for _ in range(2):
+        m = ctx.volume_metrics_dev(d_vol, d_out, nvox)
+    t = time.perf_counter()
+    for _ in range(reps):
+        m = ctx.volume_metrics_dev(d_vol, d_out, nvox)
+    ms = (time.perf_counter() - t) * 1e3 / reps
+    res = {"ms_per_call": ms, "gb_per_s": 8.0 * nvox / (ms / 1e3) / 1e9,
+           "mse": m["mse"], "mae": m["mae"], "psnr": m["psnr"], "max_abs": m["max_abs"]}
+    if vol_host is not None:
+        from oracle import oracle as O
+        n = round(nvox ** (1 / 3))
+        ref = O.metrics(vol_host, d_out.download((n, n, n)))
+        res["rel_err_vs_oracle"] = max(abs(m[k] - ref[k]) / abs(ref[k]) for k in ("mse", "mae", "psnr"))
+        res["extrema_bit_exact"] = bool(all(np.float32(m[k]) == ref[k]
+                                            for k in ("max_abs", "vmin", "vmax")))
+    log(f"[bench] metrics {res}")
+    return res
+
+
 def make_volume_sinos(ctx, seed, N, nz, na, n_ell=16):
     """Device sinograms (nz, na, N) of a seeded phantom, Serial semantics."""
     d_vol = ctx.alloc(4 * N * N * nz)
@@ -289,8 +313,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-target-s", type=float, default=15.0)
     ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--forms", default="precision,batched,served",
-                    help="extra forms to measure: precision,batched,served (or none)")
+    ap.add_argument("--forms", default="metrics,precision,batched,served",
+                    help="extra forms to measure: metrics,precision,batched,served (or none)")
     ap.add_argument("--serve-jobs", type=int, default=2000)
     ap.add_argument("--bp-config", type=int, default=-1,
                     help="fix the backprojection tile configuration (index into the "
@@ -372,6 +396,8 @@ def main():
         log(f"[bench] cpu {cpu}")
 
     forms = [f for f in args.forms.split(",") if f and f != "none"]
+    if "metrics" in forms:
+        extra["metrics_form"] = metrics_form(ctx, d_vol, d_out, nvox, vol_host)
     if "precision" in forms:
         from oracle import oracle as O
         zl = [n // 2, n // 2 + 1]

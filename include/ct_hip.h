@@ -37,7 +37,8 @@ typedef enum {
     CT_ERR_NOMEM = -3,       /* device allocation failed */
     CT_ERR_BUDGET = -4,      /* request exceeds the context's HBM budget */
     CT_ERR_UNSUPPORTED = -5, /* shape outside what the kernels support */
-    CT_ERR_NODEVICE = -6     /* no HIP device */
+    CT_ERR_NODEVICE = -6,    /* no HIP device */
+    CT_ERR_IO = -7           /* file missing, unreadable, unwritable or not in the format */
 } ct_status;
 
 /* Filter ids: the order of ReconstructionParams::FilterType (CTTypes.h:151).
@@ -236,6 +237,97 @@ int ct_hip_server_release(ct_hip_server* srv, uint64_t ticket);
  * last completion */
 int ct_hip_server_stats(ct_hip_server* srv, double* stats, int n);
 const char* ct_hip_server_last_error(const ct_hip_server* srv);
+
+/* ------------------------------------------------- reconstruction metrics */
+
+/* The producer's quality figures of a reconstruction against its original
+ * (CtReconstructionController.cpp:1199-1277), over every voxel:
+ *   diff = recon - orig (float);
+ *   mse = mean of double(diff)^2, mae = mean of |double(diff)|;
+ *   max_abs = max |diff| (float), replaced by 1 when below 1e-6, as the
+ *     controller's maxDifference;
+ *   vmin / vmax = min / max of orig;
+ *   psnr = 10 log10(range^2 / mse) with range = double(vmax - vmin), +inf
+ *     when mse <= 0 or range <= 0.
+ * The sums run on the device in double in a fixed order: the same inputs give
+ * the same bits on every call; against the controller's OpenMP sums they
+ * differ only by the order of the additions. */
+typedef struct {
+    double mse, mae, psnr;
+    float max_abs, vmin, vmax;
+    uint64_t count;
+} ct_metrics;
+
+/* orig and recon: n floats each, host memory. n = 0 gives count 0, mse and
+ * mae 0, psnr +inf, max_abs 1, vmin FLT_MAX, vmax -FLT_MAX (the controller's
+ * starting values). */
+int ct_hip_volume_metrics(ct_hip_ctx* ctx, const float* orig, const float* recon, size_t n,
+                          ct_metrics* out);
+/* Device-resident form: d_orig and d_recon in device memory, 16-byte aligned. */
+int ct_hip_volume_metrics_dev(ct_hip_ctx* ctx, const float* d_orig, const float* d_recon,
+                              size_t n, ct_metrics* out);
+
+/* ----------------------------------------------------------------- files */
+
+/* Replaces FileIO::saveVolumeNPY (FileIO.h:11; FileIO.cpp:56-82): vol (d,h,w)
+ * written as NPY 1.0 '<f4' of shape (d, h, w), byte for byte the reference's
+ * file (header padded so that 10 + its length is a multiple of 16).
+ * CT_ERR_INVALID for an empty volume, CT_ERR_IO when the file cannot be
+ * written. Needs no context and no device. */
+int ct_hip_save_volume_npy(const float* vol, size_t w, size_t h, size_t d, const char* path);
+
+/* Replaces FileIO::loadVolumeNPY (FileIO.h:12; FileIO.cpp:84-145). dims3
+ * receives {width, height, depth} of the Volume the reference builds: for
+ * input_is_xyz_layout = 0 the file's (z, y, x) order is kept (width = last
+ * axis); for 1 the file is (x, y, z) and is transposed into (z, y, x).
+ * With vol == NULL only the header is read; otherwise vol (capacity floats,
+ * at least w*h*d) receives the data. CT_ERR_IO for a missing file, a bad
+ * magic, a dtype other than '<f4', a shape without three non-zero sizes or
+ * short data. Versions 1.x and 2.x headers are read. */
+int ct_hip_load_volume_npy(const char* path, int input_is_xyz_layout, float* vol,
+                           size_t capacity, size_t* dims3);
+
+/* Replaces FileIO::savePointCloudNPY / loadPointCloudNPY (FileIO.h:14-15;
+ * FileIO.cpp:148-206): n points of 4 floats (x, y, z, hu) as an (n, 4) '<f4'
+ * NPY. The loader sets *n, and fills xyzh when it is not NULL and
+ * capacity >= *n points; a shape other than (n > 0, 4) is CT_ERR_IO. */
+int ct_hip_save_point_cloud_npy(const float* xyzh, size_t n, const char* path);
+int ct_hip_load_point_cloud_npy(const char* path, float* xyzh, size_t capacity, size_t* n);
+
+/* Sinogram stack on disk: an NPY (nz, na, bins) and, beside it, the same path
+ * with ".npy" replaced by ".meta.json" (version, depth, num_angles,
+ * detector_bins, detector_spacing_mm, original_min_hu, original_max_hu,
+ * angles_deg). */
+typedef struct {
+    size_t nz, na, bins;
+    float detector_spacing_mm; /* from the sidecar, else 1 */
+    float min_hu, max_hu;      /* from the sidecar, else the data's min/max */
+    int meta_found;            /* the sidecar was read and parsed as a JSON object */
+    int meta_minmax;           /* both min/max keys were present in it */
+    int meta_angles;           /* its angle list had na entries */
+} ct_sino_info;
+
+/* Replaces CtReconstructionController::exportAllSinograms
+ * (CtReconstructionController.cpp:1353-1482): computeSinogram of every slice
+ * of vol (d,h,w) with num_angles angles and bins = w, on the device; writes
+ * the NPY (d, num_angles, w) and the sidecar, whose angles are
+ * float(180/na) * a in double and whose min/max are the extremes of the
+ * slices' original min/max. */
+int ct_hip_export_sinograms(ct_hip_ctx* ctx, const float* vol, int w, int h, int d,
+                            int num_angles, const char* npy_path);
+
+/* Replaces CtReconstructionController::loadSinograms
+ * (CtReconstructionController.cpp:1494-1616). Two calls: with sinos == NULL
+ * only the header and sidecar are read (info filled; min/max then come from
+ * the sidecar or stay FLT_MAX / -FLT_MAX); with sinos (capacity >= nz*na*bins
+ * floats) the data is read too, and min/max fall back to the data's. Each
+ * sidecar key falls back on its own, as the reference's reader: spacing 1;
+ * angles float(180/na) * a (float product) unless the sidecar lists exactly
+ * na of them; min/max from the data unless both keys are present. A missing
+ * or unparsable sidecar uses every default. angles_deg (optional) receives
+ * na floats. Needs no context and no device. */
+int ct_hip_load_sinograms(const char* npy_path, float* sinos, size_t capacity, float* angles_deg,
+                          ct_sino_info* info);
 
 /* ------------------------------------------------- synthetic acquisition */
 

@@ -701,3 +701,42 @@ int ct_ph_rasterize(const ct_o_ellipsoid* e, int n, int nx, int ny, int nz,
     free(dv); free(xs); free(ys); free(zs);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* CtReconstructionController.cpp:1199-1277: the producer's metrics      */
+/* ------------------------------------------------------------------ */
+
+/* Sequential restatement of the controller's metrics loop and its finish:
+ * d = recon - orig (float); mse and mae accumulate double(d)^2 and
+ * |double(d)|; max |d| (float), min/max of orig (float), count. Then
+ * max_abs < 1e-6 -> 1, means over the count, range = double(vmax - vmin),
+ * psnr = +inf when mse <= 0 or range <= 0, else 10 log10(range^2 / mse).
+ * out_d = {mse, mae, psnr}; out_f = {max_abs, vmin, vmax}. */
+int ct_o_metrics(const float* orig, const float* recon, size_t n, double* out_d, float* out_f) {
+    double mse = 0.0, mae = 0.0;
+    float max_abs = 0.0f;
+    float vmin = FLT_MAX, vmax = -FLT_MAX;
+    for (size_t i = 0; i < n; ++i) {
+        const float diff = recon[i] - orig[i];
+        const float o = orig[i];
+        mse += (double)diff * (double)diff;
+        mae += fabs((double)diff);
+        /* std::max(a, b) = (a < b) ? b : a; std::min(a, b) = (b < a) ? b : a */
+        const float ad = fabsf(diff);
+        max_abs = (max_abs < ad) ? ad : max_abs;
+        vmin = (o < vmin) ? o : vmin;
+        vmax = (vmax < o) ? o : vmax;
+    }
+    if (max_abs < 1e-6f) max_abs = 1.0f;
+    mse = (n > 0) ? mse / (double)n : 0.0;
+    mae = (n > 0) ? mae / (double)n : 0.0;
+    const double range = (double)(vmax - vmin);
+    const double psnr = (mse <= 0.0 || range <= 0.0) ? INFINITY : 10.0 * log10((range * range) / mse);
+    out_d[0] = mse;
+    out_d[1] = mae;
+    out_d[2] = psnr;
+    out_f[0] = max_abs;
+    out_f[1] = vmin;
+    out_f[2] = vmax;
+    return 0;
+}

@@ -16,7 +16,24 @@
  *   ct_omp_*    restatement of OpenMPBackend::reconstructVolume, used only as
  *               the timed CPU baseline (ct_ompbase.c, built separately).
  *
- * Pinning status of each function is listed in oracle/README.md.
+ * How each function is pinned (tests/test_oracle.py, tests/test_metrics_io.py):
+ *   ct_o_forward_real / ct_o_inverse_real  bit-exact vs the reference's own
+ *       FFT.cpp built under oracle/_ref; numpy FP64 rfft; the Nyquist closed form
+ *   ct_o_create_filter     bit-exact vs the reference's FilterDesign.cpp build
+ *       (42 vectors); ramp and window closed forms
+ *   ct_o_compute_sinogram  0/90 degree closed forms (column and row sums with
+ *       the last-bin rule); mass conservation; raw-zero skip; FP64 twin
+ *   ct_o_reconstruct_slice exact padded-Toeplitz FP64 convolution (filter);
+ *       disk reconstruction position and amplitude (y-flip, scale); linearity
+ *   ct_o_reconstruct_volume composition of the above; denormalisation;
+ *       the setSlice quirk
+ *   ct_o_metrics           numpy FP64 means; constant-offset closed form;
+ *       identical/constant volumes (PSNR +inf, max 1); NaN and n = 0 cases
+ *   ct_ph_*                FP64 membership of one ellipsoid; volume closed form;
+ *       additivity; determinism
+ *   ct_o64_*               the closed forms above at tight tolerance
+ *   ct_omp_*               timing baseline only: parity unpinned (its output
+ *       is not a parity target; its reference files do not build here)
  */
 #ifndef CT_ORACLE_H
 #define CT_ORACLE_H
@@ -75,6 +92,10 @@ int ct_o64_reconstruct_slice(const double* sino, size_t na, size_t bins,
 int ct_o64_reconstruct_volume(const float* vol, size_t w, size_t h, size_t d,
                               size_t na, int filter, const int* z_list, size_t nz_list,
                               double* out, int threads);
+
+/* ---- CtReconstructionController.cpp:1199-1277 metrics ---- */
+/* out_d = {mse, mae, psnr}; out_f = {max_abs (1 when < 1e-6), vmin, vmax} */
+int ct_o_metrics(const float* orig, const float* recon, size_t n, double* out_d, float* out_f);
 
 /* ---- project phantom ---- */
 typedef struct {

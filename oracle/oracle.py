@@ -66,6 +66,7 @@ def lib():
         L.ct_o64_reconstruct_slice.argtypes = [_f64p, _sz, _sz, _f64p, _sz, ctypes.c_int, _f64p, _f64p]
         L.ct_o64_reconstruct_volume.argtypes = [_f32p, _sz, _sz, _sz, _sz, ctypes.c_int, _i32p, _sz,
                                                 _f64p, ctypes.c_int]
+        L.ct_o_metrics.argtypes = [_f32p, _f32p, _sz, _f64p, _f32p]
         L.ct_ph_params.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(Ellipsoid)]
         L.ct_ph_rasterize.argtypes = [ctypes.POINTER(Ellipsoid), ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_int]
@@ -281,6 +282,18 @@ def omp_reconstruct_volume(vol, na: int, ftype: int, threads: int):
     _check(ompbase().ct_omp_reconstruct_volume(_p32(vol), w, h, d, na, ftype, _p32(out), threads,
                                                _p64(ph)), "omp_reconstruct_volume")
     return out, ph
+
+
+def metrics(orig, recon) -> dict:
+    """CtReconstructionController.cpp:1199-1277 metrics, sequential (ct_o_metrics)."""
+    o, r = _c32(orig).ravel(), _c32(recon).ravel()
+    assert o.size == r.size
+    d = np.empty(3, np.float64)
+    f = np.empty(3, np.float32)
+    _check(lib().ct_o_metrics(_p32(o), _p32(r), o.size, _p64(d), _p32(f)), "metrics")
+    return {"mse": float(d[0]), "mae": float(d[1]), "psnr": float(d[2]),
+            "max_abs": np.float32(f[0]), "vmin": np.float32(f[1]), "vmax": np.float32(f[2]),
+            "count": o.size}
 
 
 def rel_l2(a, b) -> float:

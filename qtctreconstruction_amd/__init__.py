@@ -45,8 +45,8 @@ class BackendError(RuntimeError):
         self.code = code
 
 
-CT_OK, CT_ERR_INVALID, CT_ERR_HIP, CT_ERR_NOMEM, CT_ERR_BUDGET, CT_ERR_UNSUPPORTED, CT_ERR_NODEVICE = (
-    0, -1, -2, -3, -4, -5, -6)
+(CT_OK, CT_ERR_INVALID, CT_ERR_HIP, CT_ERR_NOMEM, CT_ERR_BUDGET, CT_ERR_UNSUPPORTED, CT_ERR_NODEVICE,
+ CT_ERR_IO) = (0, -1, -2, -3, -4, -5, -6, -7)
 
 
 class Ellipsoid(ctypes.Structure):
@@ -64,6 +64,24 @@ class FbpJob(ctypes.Structure):
     _fields_ = [("d_sinos", ctypes.c_void_p), ("d_out", ctypes.c_void_p),
                 ("angles_deg", ctypes.POINTER(ctypes.c_float)), ("nz", ctypes.c_int),
                 ("na", ctypes.c_int), ("bins", ctypes.c_int), ("filter", ctypes.c_int)]
+
+
+class Metrics(ctypes.Structure):
+    """ct_metrics (include/ct_hip.h): the controller's reconstruction metrics."""
+    _fields_ = [("mse", ctypes.c_double), ("mae", ctypes.c_double), ("psnr", ctypes.c_double),
+                ("max_abs", ctypes.c_float), ("vmin", ctypes.c_float), ("vmax", ctypes.c_float),
+                ("count", ctypes.c_uint64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SinoInfo(ctypes.Structure):
+    """ct_sino_info (include/ct_hip.h): shape and sidecar fields of a sinogram file."""
+    _fields_ = [("nz", ctypes.c_size_t), ("na", ctypes.c_size_t), ("bins", ctypes.c_size_t),
+                ("detector_spacing_mm", ctypes.c_float), ("min_hu", ctypes.c_float),
+                ("max_hu", ctypes.c_float), ("meta_found", ctypes.c_int),
+                ("meta_minmax", ctypes.c_int), ("meta_angles", ctypes.c_int)]
 
 
 CT_BATCH_GRAPH, CT_BATCH_AUTOTUNE = 1, 2
@@ -129,6 +147,14 @@ def load_library(path: str = LIB_PATH):
         "ct_hip_last_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double)]),
         "ct_hip_last_radon_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int)]),
+        "ct_hip_volume_metrics": (i, [ctx, _f32p, _f32p, sz, ctypes.POINTER(Metrics)]),
+        "ct_hip_volume_metrics_dev": (i, [ctx, _vp, _vp, sz, ctypes.POINTER(Metrics)]),
+        "ct_hip_save_volume_npy": (i, [_f32p, sz, sz, sz, ctypes.c_char_p]),
+        "ct_hip_load_volume_npy": (i, [ctypes.c_char_p, i, _f32p, sz, ctypes.POINTER(sz)]),
+        "ct_hip_save_point_cloud_npy": (i, [_f32p, sz, ctypes.c_char_p]),
+        "ct_hip_load_point_cloud_npy": (i, [ctypes.c_char_p, _f32p, sz, ctypes.POINTER(sz)]),
+        "ct_hip_export_sinograms": (i, [ctx, _f32p, i, i, i, i, ctypes.c_char_p]),
+        "ct_hip_load_sinograms": (i, [ctypes.c_char_p, _f32p, sz, _f32p, ctypes.POINTER(SinoInfo)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -155,7 +181,9 @@ def exported_symbols() -> List[str]:
         "ct_hip_server_download", "ct_hip_server_release", "ct_hip_server_stats",
         "ct_hip_server_last_error", "ct_hip_radon_dev", "ct_hip_noisy_log_dev",
         "ct_hip_last_timings",
-        "ct_hip_last_radon_timings",
+        "ct_hip_last_radon_timings", "ct_hip_volume_metrics", "ct_hip_volume_metrics_dev",
+        "ct_hip_save_volume_npy", "ct_hip_load_volume_npy", "ct_hip_save_point_cloud_npy",
+        "ct_hip_load_point_cloud_npy", "ct_hip_export_sinograms", "ct_hip_load_sinograms",
     ]
 
 
@@ -348,6 +376,13 @@ class Context:
         self._call("ct_hip_phantom_dev", self.handle, e, n, nx, ny, nz, buf.ptr)
         return buf
 
+    def volume_metrics_dev(self, d_orig: DeviceBuffer, d_recon: DeviceBuffer, n: int) -> dict:
+        """Reconstruction metrics of n voxels held in device memory (ct_metrics as a dict)."""
+        m = Metrics()
+        self._call("ct_hip_volume_metrics_dev", self.handle, d_orig.ptr, d_recon.ptr, int(n),
+                   ctypes.byref(m))
+        return m.as_dict()
+
 
 class Server:
     """Serving form (ct_hip_server_*): continuous batching over device-resident sinograms."""
@@ -515,8 +550,136 @@ class HipBackend:
                            float(threshold), _p(out), n.value, ctypes.byref(n))
         return out
 
+    def volumeMetrics(self, original, reconstruction) -> dict:
+        """The controller's metrics of a reconstruction against its original
+        (CtReconstructionController.cpp:1199-1277): mse, mae, psnr, max_abs,
+        vmin, vmax, count. Both volumes must hold the same number of voxels."""
+        o, r = _f32(original), _f32(reconstruction)
+        if o.size != r.size:
+            raise ValueError("volumeMetrics: volumes differ in size")
+        m = Metrics()
+        self.ctx._call("ct_hip_volume_metrics", self.ctx.handle, _p(o), _p(r), o.size,
+                       ctypes.byref(m))
+        return m.as_dict()
+
     def lastSinogramTimeMs(self) -> float:
         return self._sino_ms
 
     def lastReconstructionTimeMs(self) -> float:
         return self._recon_ms
+
+
+def _path(p) -> bytes:
+    return os.fsencode(os.fspath(p))
+
+
+class FileIO:
+    """FileIO (FileIO.h:9-21): NPY volumes and point clouds, in the
+    reference's byte format. Loaders return None where the reference returns
+    false; savers return the reference's bool. Slice PNG/BMP export belongs to
+    the UI and is not provided."""
+
+    @staticmethod
+    def saveVolumeNPY(volume, path) -> bool:
+        v = _f32(volume)
+        if v.ndim != 3 or v.size == 0:
+            return False
+        d, h, w = v.shape
+        return load_library().ct_hip_save_volume_npy(_p(v), w, h, d, _path(path)) == CT_OK
+
+    @staticmethod
+    def loadVolumeNPY(path, input_is_xyz_layout: bool = False) -> Optional[np.ndarray]:
+        """Volume (depth, height, width) as the reference builds it (FileIO.cpp:84-145).
+        Its axis coordinates are linspace(-100, 100, n) per axis (volume_coords)."""
+        L = load_library()
+        dims = (ctypes.c_size_t * 3)()
+        if L.ct_hip_load_volume_npy(_path(path), int(input_is_xyz_layout), None, 0, dims) != CT_OK:
+            return None
+        w, h, d = dims[0], dims[1], dims[2]
+        out = np.empty((d, h, w), np.float32)
+        if L.ct_hip_load_volume_npy(_path(path), int(input_is_xyz_layout), _p(out), out.size,
+                                    dims) != CT_OK:
+            return None
+        return out
+
+    @staticmethod
+    def savePointCloudNPY(cloud, path) -> bool:
+        c = _f32(cloud).reshape(-1, 4) if np.size(cloud) else np.zeros((0, 4), np.float32)
+        if c.shape[0] == 0:
+            return False
+        return load_library().ct_hip_save_point_cloud_npy(_p(c), c.shape[0], _path(path)) == CT_OK
+
+    @staticmethod
+    def loadPointCloudNPY(path) -> Optional[np.ndarray]:
+        L = load_library()
+        n = ctypes.c_size_t(0)
+        if L.ct_hip_load_point_cloud_npy(_path(path), None, 0, ctypes.byref(n)) != CT_OK:
+            return None
+        out = np.empty((n.value, 4), np.float32)
+        if L.ct_hip_load_point_cloud_npy(_path(path), _p(out), n.value, ctypes.byref(n)) != CT_OK:
+            return None
+        return out
+
+
+def volume_coords(n: int) -> np.ndarray:
+    """Axis coordinates a loaded volume carries: linspace(-100, 100, n) in
+    float (FileIO.cpp:141-143; Utils linspace: start + i*step, step = (b-a)/(n-1))."""
+    if n <= 0:
+        return np.zeros(0, np.float32)
+    if n == 1:
+        return np.array([-100.0], np.float32)
+    step = np.float32(200.0) / np.float32(n - 1)
+    return (np.float32(-100.0) + np.arange(n, dtype=np.float32) * step).astype(np.float32)
+
+
+@dataclass
+class LoadedSinograms:
+    """What CtReconstructionController::loadSinograms keeps
+    (CtReconstructionController.cpp:1600-1607)."""
+    sinograms: np.ndarray          # (nz, na, bins)
+    angles_deg: np.ndarray         # na floats
+    detector_spacing_mm: float
+    min_hu: float
+    max_hu: float
+    meta_found: bool
+    meta_minmax: bool
+    meta_angles: bool
+
+
+def exportAllSinograms(backend: "HipBackend", volume, num_angles: int, npy_path) -> bool:
+    """CtReconstructionController::exportAllSinograms
+    (CtReconstructionController.cpp:1353-1482) for a volume in memory: the
+    sinograms of every slice (bins = width) computed on the device, written as
+    NPY (depth, num_angles, width) plus the .meta.json sidecar. Raises
+    BackendError on a device failure; returns False when the files cannot be
+    written, as the reference does."""
+    v = _f32(volume)
+    if v.ndim != 3 or v.size == 0 or num_angles <= 0:
+        return False
+    d, h, w = v.shape
+    rc = backend.ctx.lib.ct_hip_export_sinograms(backend.ctx.handle, _p(v), w, h, d,
+                                                 int(num_angles), _path(npy_path))
+    if rc == CT_ERR_IO:
+        return False
+    if rc != CT_OK:
+        msg = backend.ctx.lib.ct_hip_last_error(backend.ctx.handle) or b""
+        raise BackendError(rc, f"ct_hip_export_sinograms: {msg.decode(errors='replace')}")
+    return True
+
+
+def loadSinograms(npy_path) -> Optional[LoadedSinograms]:
+    """CtReconstructionController::loadSinograms
+    (CtReconstructionController.cpp:1494-1616) without the file dialog: the
+    sinogram stack and its sidecar fields with the reader's fallbacks. None
+    where the reference returns false. Host only; needs no device."""
+    L = load_library()
+    info = SinoInfo()
+    if L.ct_hip_load_sinograms(_path(npy_path), None, 0, None, ctypes.byref(info)) != CT_OK:
+        return None
+    s = np.empty((info.nz, info.na, info.bins), np.float32)
+    ang = np.empty(info.na, np.float32)
+    if L.ct_hip_load_sinograms(_path(npy_path), _p(s), s.size, _p(ang), ctypes.byref(info)) != CT_OK:
+        return None
+    return LoadedSinograms(s, ang, float(info.detector_spacing_mm), float(info.min_hu),
+                           float(info.max_hu), bool(info.meta_found), bool(info.meta_minmax),
+                           bool(info.meta_angles))

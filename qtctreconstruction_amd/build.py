@@ -27,7 +27,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract
           "-I", os.path.join(os.path.dirname(PKG), "include")]
 
 SOURCES = ["phantom.hip", "radon.hip", "filter.hip", "backproject.hip", "pointcloud.hip",
-           "filter_gemm.hip", "filter_design.cpp", "capi.cpp", "batch.cpp", "serve.cpp"]
+           "filter_gemm.hip", "filter_design.cpp", "metrics.hip", "fileio.cpp", "capi.cpp",
+           "batch.cpp", "serve.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -62,7 +63,7 @@ DRIVER = os.path.join(PKG, "hip_backend_driver")
 
 def build_driver() -> str:
     """The C++ IReconstructionBackend adapter + its driver (host code only, g++)."""
-    srcs = [os.path.join(HOST, f) for f in ("HipBackend.cpp", "BackendFactory.cpp",
+    srcs = [os.path.join(HOST, f) for f in ("HipBackend.cpp", "BackendFactory.cpp", "FileIO.cpp",
                                              "hip_backend_driver.cpp")]
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", DRIVER + ".tmp", *srcs,
            "-L", PKG, "-lct_hip", "-Wl,-rpath,$ORIGIN"]

@@ -4,7 +4,9 @@
 // reference's backend calls (IReconstructionBackend.h:10-55).
 #include "ctx.h"
 
+#include <cfloat>
 #include <cmath>
+#include <limits>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -422,7 +424,7 @@ int ct_hip_clear_workspace(ct_hip_ctx* c) {
     return guarded(c, [&] {
         hip_check(hipStreamSynchronize(c->stream), "sync");
         for (Buf* b : {&c->vol, &c->row, &c->col, &c->sino, &c->filt, &c->out, &c->keys, &c->mm,
-                       &c->ell, &c->pc_cnt, &c->pc_pts, &c->pc_coord})
+                       &c->ell, &c->pc_cnt, &c->pc_pts, &c->pc_coord, &c->met})
             release(c, *b);
         for (auto& kv : c->trig) {
             dfree(c, kv.second.cos);
@@ -747,6 +749,96 @@ int ct_hip_extract_point_cloud(ct_hip_ctx* c, const float* vol, int w, int h, in
         hip_check(hipMemcpyAsync(d_vol, vol, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
                   "upload volume");
         point_cloud_dev(c, d_vol, w, h, d, xc, yc, zc, threshold, out, cap, count);
+    });
+}
+
+// CtReconstructionController.cpp:1199-1277: the sums come from the device
+// pass; the means, PSNR and the max-difference rule are the controller's
+// double arithmetic, in its order.
+static void metrics_dev(ct_hip_ctx* c, const float* d_orig, const float* d_recon, size_t n,
+                        ct_metrics* out) {
+    need(out != nullptr, "volumeMetrics: null output");
+    MetricsSums m{0.0, 0.0, 0.0f, FLT_MAX, -FLT_MAX, 0.0f};
+    if (n > 0) {
+        need(d_orig && d_recon, "volumeMetrics: null pointer");
+        need(((uintptr_t)d_orig & 15) == 0 && ((uintptr_t)d_recon & 15) == 0,
+             "volumeMetrics: device pointers must be 16-byte aligned");
+        MetricsSums* part =
+            ensure<MetricsSums>(c, c->met, sizeof(MetricsSums) * (kMetricsBlocks + 1));
+        hip_check(launch_metrics(d_orig, d_recon, n, part, part + kMetricsBlocks, c->stream),
+                  "metrics");
+        hip_check(hipMemcpyAsync(&m, part + kMetricsBlocks, sizeof(m), hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "download metrics");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+    }
+    float max_abs = m.mx;
+    if (max_abs < 1e-6f) max_abs = 1.0f;
+    const double mse = n > 0 ? m.s2 / (double)n : 0.0;
+    const double mae = n > 0 ? m.s1 / (double)n : 0.0;
+    const double range = (double)(m.hi - m.lo);
+    out->mse = mse;
+    out->mae = mae;
+    out->psnr = (mse <= 0.0 || range <= 0.0) ? std::numeric_limits<double>::infinity()
+                                             : 10.0 * std::log10((range * range) / mse);
+    out->max_abs = max_abs;
+    out->vmin = m.lo;
+    out->vmax = m.hi;
+    out->count = (uint64_t)n;
+}
+
+int ct_hip_volume_metrics_dev(ct_hip_ctx* c, const float* d_orig, const float* d_recon, size_t n,
+                              ct_metrics* out) {
+    return guarded(c, [&] { metrics_dev(c, d_orig, d_recon, n, out); });
+}
+
+int ct_hip_volume_metrics(ct_hip_ctx* c, const float* orig, const float* recon, size_t n,
+                          ct_metrics* out) {
+    return guarded(c, [&] {
+        if (n == 0) return metrics_dev(c, nullptr, nullptr, 0, out);
+        need(orig && recon, "volumeMetrics: null pointer");
+        float* d_o = ensure<float>(c, c->vol, sizeof(float) * n);
+        float* d_r = ensure<float>(c, c->out, sizeof(float) * n);
+        hip_check(hipMemcpyAsync(d_o, orig, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
+                  "upload original");
+        hip_check(hipMemcpyAsync(d_r, recon, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
+                  "upload reconstruction");
+        metrics_dev(c, d_o, d_r, n, out);
+    });
+}
+
+// CtReconstructionController.cpp:1353-1482 (exportAllSinograms): the
+// sinogram of every slice with num_angles angles and bins = w, written as
+// (d, num_angles, w) with the sidecar; the sidecar's min/max are the extremes
+// of the slices' original min/max.
+int ct_hip_export_sinograms(ct_hip_ctx* c, const float* vol, int w, int h, int d, int na,
+                            const char* npy_path) {
+    return guarded(c, [&] {
+        need(vol && npy_path, "exportSinograms: null pointer");
+        need(w > 0 && h > 0 && d > 0 && na > 0, "exportSinograms: empty input");
+        const size_t pix = (size_t)w * h * d, ns = (size_t)d * na * w;
+        float* d_vol = ensure<float>(c, c->vol, sizeof(float) * pix);
+        float* d_sino = ensure<float>(c, c->sino, sizeof(float) * ns);
+        float* d_mm = ensure<float>(c, c->mm, sizeof(float) * 2 * (size_t)d);
+        const std::vector<float> ang = serial_angles(na);
+        set_trig(c, ang.data(), na);
+        hip_check(hipMemcpyAsync(d_vol, vol, sizeof(float) * pix, hipMemcpyHostToDevice, c->stream),
+                  "upload volume");
+        stage_radon(c, d_vol, w, h, d, na, w, d_sino, d_mm);
+        std::vector<float> sino(ns), mm(2 * (size_t)d);
+        hip_check(hipMemcpyAsync(sino.data(), d_sino, sizeof(float) * ns, hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "download sinograms");
+        hip_check(hipMemcpyAsync(mm.data(), d_mm, sizeof(float) * mm.size(), hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "download min/max");
+        hip_check(hipStreamSynchronize(c->stream), "sync");
+        float lo = FLT_MAX, hi = -FLT_MAX;
+        for (int z = 0; z < d; ++z) {
+            if (mm[2 * z] < lo) lo = mm[2 * z];
+            if (mm[2 * z + 1] > hi) hi = mm[2 * z + 1];
+        }
+        write_sinogram_files(npy_path, sino.data(), d, na, w, lo, hi);
     });
 }
 

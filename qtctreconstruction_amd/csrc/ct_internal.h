@@ -206,4 +206,18 @@ hipError_t launch_pc_write(const float* d_v, size_t n, int w, int h, float thr,
                            const unsigned* d_offs, const float* d_xc, const float* d_yc,
                            const float* d_zc, float4* d_out, hipStream_t s);
 
+// ---- reconstruction metrics ---------------------------------------------------
+// Sums and extrema of diff = recon - orig over a volume: s2 = sum diff^2,
+// s1 = sum |diff| (double), mx = max |diff|, lo/hi = min/max of orig (float).
+struct MetricsSums {
+    double s2, s1;
+    float mx, lo, hi, pad;
+};
+// Most partial blocks of the metrics pass (the partial buffer's length).
+constexpr int kMetricsBlocks = 1024;
+int metrics_blocks(size_t n);
+// d_orig and d_recon 16-byte aligned; d_part holds metrics_blocks(n) entries.
+hipError_t launch_metrics(const float* d_orig, const float* d_recon, size_t n,
+                          MetricsSums* d_part, MetricsSums* d_out, hipStream_t s);
+
 }  // namespace cthip

@@ -67,7 +67,7 @@ struct ct_hip_ctx {
     std::string err;
     std::unordered_map<void*, size_t> sizes;  // every live device allocation
     // workspaces (grow-only, reused across calls)
-    Buf vol, row, col, sino, filt, out, keys, mm, ell, pc_cnt, pc_pts, pc_coord;
+    Buf vol, row, col, sino, filt, out, keys, mm, ell, pc_cnt, pc_pts, pc_coord, met;
     // cached device state
     std::map<std::pair<int, int>, float*> filters;  // (nfft, type) -> H[nfft/2+1]
     std::map<int, float2*> twiddles;                // nfft -> tw[nfft-1], stage order
@@ -125,6 +125,8 @@ void slot_release(ct_hip_ctx* c, BatchSlot* s);
 size_t job_filt_bytes(const ct_fbp_job& j);
 void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj,
                    float4* const* filts, int flags);
+void write_sinogram_files(const std::string& npy_path, const float* sinos, int nz, int na,
+                          int bins, float min_hu, float max_hu);
 
 template <typename T>
 T* ensure(ct_hip_ctx* c, Buf& b, size_t bytes) {

@@ -132,4 +132,22 @@ PointCloud HipBackend::extractPointCloud(const Volume& vol, float threshold) {
     return cloud;
 }
 
+HipBackend::Metrics HipBackend::volumeMetrics(const Volume& o, const Volume& r) {
+    if (o.data.size() != r.data.size())
+        throw std::invalid_argument("volumeMetrics: volumes differ in size");
+    if (!m_ctx) check(CT_ERR_NODEVICE, "volumeMetrics");
+    ct_metrics m{};
+    check(ct_hip_volume_metrics(m_ctx, o.data.data(), r.data.data(), o.data.size(), &m),
+          "volumeMetrics");
+    Metrics out;
+    out.mse = m.mse;
+    out.mae = m.mae;
+    out.psnr = m.psnr;
+    out.max_abs = m.max_abs;
+    out.vmin = m.vmin;
+    out.vmax = m.vmax;
+    out.count = (size_t)m.count;
+    return out;
+}
+
 }  // namespace ct

@@ -37,6 +37,17 @@ public:
     double lastSinogramTimeMs() const override { return m_lastSinogramTimeMs; }
     double lastReconstructionTimeMs() const override { return m_lastReconstructionTimeMs; }
 
+    // Not virtuals of the interface: the controller's reconstruction metrics
+    // (CtReconstructionController.cpp:1199-1277) computed on the device, and
+    // the C-ABI context for the file helpers of FileIO.h.
+    struct Metrics {
+        double mse = 0, mae = 0, psnr = 0;
+        float max_abs = 0, vmin = 0, vmax = 0;
+        size_t count = 0;
+    };
+    Metrics volumeMetrics(const Volume& original, const Volume& reconstruction);
+    ct_hip_ctx* context() const { return m_ctx; }
+
 private:
     void check(int rc, const char* what) const;  // throws std::runtime_error, as GPUCHECK does
     ct_hip_ctx* m_ctx = nullptr;

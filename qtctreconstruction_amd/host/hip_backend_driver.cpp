@@ -1,9 +1,11 @@
 // hip_backend_driver.cpp — drives HipBackend through the IReconstructionBackend
 // interface obtained from BackendFactory, the way the reference's controller
 // does (CtReconstructionController.cpp:1096-1146, 984-1003, 629-634), and
-// writes every result as raw float32 files for the parity test.
+// writes every result as raw float32 files for the parity test. The file
+// formats (FileIO.h) and the device metrics are driven the same way.
 //
 // usage: hip_backend_driver <volume.f32> <w> <h> <d> <num_angles> <filter> <threshold> <outdir>
+#include "FileIO.h"
 #include "HipBackend.h"
 
 #include <cstdio>
@@ -74,6 +76,27 @@ int main(int argc, char** argv) {
         PointCloud pc = backend->extractPointCloud(vol, thr);
         dump(out + "/points.f32", reinterpret_cast<const float*>(pc.data()), pc.size() * 4);
 
+        // files: the sinogram stack exported and loaded back, then the
+        // consumer path from the loaded stack; volume and point cloud NPYs
+        HipBackend* hb = dynamic_cast<HipBackend*>(backend.get());
+        if (!hb) throw std::runtime_error("best available backend is not HipBackend");
+        if (!SinogramFiles::exportAll(*hb, vol, na, out + "/sinos.npy"))
+            throw std::runtime_error("exportAll failed");
+        LoadedSinograms ls;
+        if (!SinogramFiles::load(out + "/sinos.npy", ls)) throw std::runtime_error("load failed");
+        Volume rec3;
+        backend->reconstructVolumeFromSinograms(ls.sinograms, ls.angles_deg, rec3, params, nullptr);
+        dump(out + "/fromfile.f32", rec3.data.data(), rec3.data.size());
+        Volume back;
+        const bool vol_rt = FileIO::saveVolumeNPY(rec, out + "/volume.npy") &&
+                            FileIO::loadVolumeNPY(out + "/volume.npy", back, false) &&
+                            back.data == rec.data && back.x_coords.size() == w;
+        PointCloud pc_back;
+        const bool pc_rt = pc.empty() || (FileIO::savePointCloudNPY(pc, out + "/points.npy") &&
+                                          FileIO::loadPointCloudNPY(out + "/points.npy", pc_back) &&
+                                          pc_back.size() == pc.size());
+        const HipBackend::Metrics m = hb->volumeMetrics(vol, rec);
+
         // empty inputs give empty results, no exception
         Sinogram e = backend->computeSinogram(Buffer2D(), na, w);
         Buffer2D er = backend->reconstructSlice(Sinogram(), w, params);
@@ -84,7 +107,12 @@ int main(int argc, char** argv) {
              << pc.size() << "\n"
              << (e.data.empty() && er.empty()) << "\n"
              << backend->lastSinogramTimeMs() << "\n"
-             << backend->lastReconstructionTimeMs() << "\n";
+             << backend->lastReconstructionTimeMs() << "\n"
+             << vol_rt << "\n"
+             << pc_rt << "\n";
+        meta.precision(17);
+        meta << m.mse << "\n" << m.mae << "\n" << m.psnr << "\n" << m.max_abs << "\n" << m.vmin
+             << "\n" << m.vmax << "\n" << m.count << "\n";
     } catch (const std::exception& ex) {
         std::cerr << "exception: " << ex.what() << "\n";
         return 5;

@@ -307,3 +307,15 @@ def test_cpp_hip_backend_through_the_interface(O, tmp_path):
     pts = np.fromfile(tmp_path / "points.f32", np.float32).reshape(-1, 4)
     xc = 0.5 * np.arange(w, dtype=np.float32)
     assert _bits_equal(pts, _serial_point_cloud(vol, thr, xc=xc))
+    # files: exported + reloaded stack reconstructs exactly as the in-memory one
+    # (na = 90: the sidecar's angles equal the float product 2.0f * a)
+    ff = np.fromfile(tmp_path / "fromfile.f32", np.float32).reshape(d, w, w)
+    assert _bits_equal(ff, fs)
+    assert meta[6] == "1" and meta[7] == "1"  # volume and point-cloud NPY round trips
+    np.testing.assert_array_equal(np.load(tmp_path / "volume.npy"), v)
+    m = O.metrics(vol, v)
+    got = [float(x) for x in meta[8:15]]
+    for k, g in zip(("mse", "mae", "psnr"), got[:3]):
+        assert g == pytest.approx(m[k], rel=1e-9), k
+    assert np.float32(got[3]) == m["max_abs"] and np.float32(got[4]) == m["vmin"]
+    assert np.float32(got[5]) == m["vmax"] and int(got[6]) == vol.size
