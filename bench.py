@@ -258,9 +258,12 @@ def served_form(ctx, n_jobs, seed=5, pool_gb=96.0, rate_factor=0.7):
     lam = rate_factor / mean_job_s
     log(f"[bench] serving: plain {plain_ups:.3e} upd/s, {1/mean_job_s:.1f} jobs/s -> rate {lam:.1f}/s")
 
-    def run(open_loop):
+    poisson_arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs))
+
+    def run(open_loop, policy=ct.SERVE_SIZE_AGED, cap_ms=5.0):
         srv = ct.Server(pool_bytes=int(pool_gb * (1 << 30)), max_batch=32, slots=2)
-        arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs)) if open_loop else np.zeros(n_jobs)
+        srv.set_policy(policy, batch_cap_ms=cap_ms)
+        arrivals = poisson_arrivals if open_loop else np.zeros(n_jobs)
         lat = []
         done = 0
         start = time.perf_counter()
@@ -293,11 +296,17 @@ def served_form(ctx, n_jobs, seed=5, pool_gb=96.0, rate_factor=0.7):
     log("[bench] serving: Poisson arrivals")
     poisson = run(True)
     log(f"[bench] serving poisson: {poisson}")
+    log("[bench] serving: the same Poisson arrivals, FIFO admission, no batch cap")
+    poisson_fifo = run(True, ct.SERVE_FIFO, 0.0)
+    log(f"[bench] serving poisson fifo: {poisson_fifo}")
     for d_s in pool.values():
         d_s.free()
     return {"jobs": n_jobs, "rate_per_s": lam, "rate_factor_of_plain": rate_factor,
             "plain_per_job_voxel_updates_per_s": plain_ups, "saturated": saturated,
-            "poisson": poisson, "latency": "submit -> completion in device memory",
+            "poisson": poisson, "poisson_fifo": poisson_fifo,
+            "policy": "size-ordered with waiting credit 0.25, batch cap max(5 ms, queue/4); "
+                      "poisson_fifo: FIFO without a cap on the same arrivals",
+            "latency": "submit -> completion in device memory",
             "noise": f"counts ~ Poisson(1e4 exp(-kappa p)) (normal approx.), kappa = 2/N, log"}
 
 

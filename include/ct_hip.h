@@ -212,15 +212,30 @@ int ct_hip_set_bp_config(ct_hip_ctx* ctx, int config);
 /* A request loop over device-resident sinograms with continuous batching
  * (no reference counterpart). The server owns a context and one device pool
  * of pool_bytes for outputs and filtered-sinogram workspaces; jobs are
- * admitted FIFO when both fit, batched up to max_batch, and run in up to
- * `slots` batches in flight on separate streams. Each job's result equals
- * its standalone reconstructVolumeFromSinograms result bit for bit. */
+ * admitted in the policy's order when both fit (the first that does not fit
+ * stops admission), batched up to max_batch, and run in up to `slots`
+ * batches in flight on separate streams. Each job's result equals its
+ * standalone reconstructVolumeFromSinograms result bit for bit, whatever the
+ * policy and the mix. */
 typedef struct ct_hip_server ct_hip_server;
 int ct_hip_server_create(int device, size_t pool_bytes, int max_batch, int slots,
                          ct_hip_server** out);
 void ct_hip_server_destroy(ct_hip_server* srv);
 /* CT_BATCH_* flags for the server's batches (default graph + autotune). */
 int ct_hip_server_set_flags(ct_hip_server* srv, int flags);
+/* Admission policy:
+ *   CT_SERVE_FIFO        submission order;
+ *   CT_SERVE_SIZE_AGED   (default) ascending estimated device time minus
+ *                        age_weight x time waited (both in ms; default 0.25),
+ *                        so small jobs pass large ones and a large job's
+ *                        credit grows while it waits.
+ * batch_cap_ms: a batch stops growing once its estimated device time would
+ * pass this or a quarter of the queued work's, whichever is more (default
+ * 5 ms; 0 = no cap; a batch always takes one job). The estimates use the
+ * device rate measured over the server's completed batches. */
+enum { CT_SERVE_FIFO = 0, CT_SERVE_SIZE_AGED = 1 };
+int ct_hip_server_set_policy(ct_hip_server* srv, int policy, double batch_cap_ms,
+                             double age_weight);
 /* job->d_sinos must stay valid until the job is finished; job->d_out is
  * ignored (the output is placed in the pool). angles are copied. */
 int ct_hip_server_submit(ct_hip_server* srv, const ct_fbp_job* job, uint64_t* ticket);

@@ -3,8 +3,17 @@
 //
 //   * Admission by HBM budget: the server owns one device pool (a first-fit
 //     sub-allocator over a single allocation); a job is admitted when its
-//     filtered-sinogram workspace and its output fit. Admission is FIFO: the
-//     first job that does not fit stops admission, so large jobs never starve.
+//     filtered-sinogram workspace and its output fit. The first job in
+//     admission order that does not fit stops admission, so a large job is
+//     not passed over once it reaches the head.
+//   * Admission order (ct_hip_server_set_policy): FIFO, or by estimated
+//     device time minus a credit for time waited (the default). A batch
+//     stops growing at a cap on its estimated device time (or a quarter of
+//     the queued work, if more): the jobs of a batch complete together, so a
+//     small job batched with a large one would wait for it. Ordering by size lets the many small jobs pass the few
+//     large ones; the waiting credit bounds how long a large job can be
+//     passed over. The device rate behind the estimates is measured: updates
+//     completed per millisecond of time with a batch in flight.
 //   * Continuous batching: up to `slots` batches are in flight at once, each
 //     on its own stream. Jobs that arrive while batches run are admitted into
 //     the next free slot as soon as memory frees up, so their persistent
@@ -18,6 +27,7 @@
 // Latency = submit -> completion observed by the worker (host clock).
 #include "ctx.h"
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -94,6 +104,7 @@ struct Job {
     ct_fbp_job d{};
     std::vector<float> angles;
     size_t out_off = 0, filt_off = 0, out_bytes = 0, filt_bytes = 0;
+    double updates = 0;  // nz * bins^2 * na voxel-updates
     double t_submit = 0, t_admit = 0, t_done = 0;
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
     int slot = -1;
@@ -114,6 +125,11 @@ struct ct_hip_server {
     int max_batch = 16;
     int flags = CT_BATCH_GRAPH | CT_BATCH_AUTOTUNE;
     std::vector<Slot> slots;
+    // admission policy (ct_hip_server_set_policy)
+    int policy = CT_SERVE_SIZE_AGED;
+    double batch_cap_ms = 5.0, age_weight = 0.25;
+    // device rate estimate: updates completed per ms with a batch in flight
+    double busy_ms = 0, busy_start = -1, done_updates = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<Job*> queue;
@@ -137,12 +153,42 @@ void fail_job(ct_hip_server* s, Job* j, const std::string& msg) {
     s->done.push_back(j);
 }
 
-// Called with s->mu held. Admits queued jobs FIFO into slot k and launches.
+// Updates per ms: measured once 20 ms of busy time have been seen, else a
+// figure typical of a full device (5e12 voxel-updates/s).
+double rate_per_ms(const ct_hip_server* s) {
+    return s->busy_ms > 20.0 ? s->done_updates / s->busy_ms : 5e9;
+}
+
+// Puts the queue in admission order (stable, so FIFO among equal keys).
+void order_queue(ct_hip_server* s) {
+    if (s->policy != CT_SERVE_SIZE_AGED || s->queue.size() < 2) return;
+    const double t = now_ms(), r = rate_per_ms(s), a = s->age_weight;
+    std::vector<std::pair<double, Job*>> key;
+    key.reserve(s->queue.size());
+    for (Job* j : s->queue) key.emplace_back(j->updates / r - a * (t - j->t_submit), j);
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<double, Job*>& x, const std::pair<double, Job*>& y) {
+                         return x.first < y.first;
+                     });
+    for (size_t i = 0; i < key.size(); ++i) s->queue[i] = key[i].second;
+}
+
+// Called with s->mu held. Admits queued jobs, in admission order, into slot k
+// and launches them as one batch.
 bool launch_into(ct_hip_server* s, int k, std::unique_lock<std::mutex>& lk) {
     Slot& sl = s->slots[k];
     std::vector<Job*> batch;
+    order_queue(s);
+    // the cap grows with the backlog (a quarter of the queued work), so a long
+    // queue drains in large batches, which keep the device fuller
+    double queued = 0;
+    for (const Job* j : s->queue) queued += j->updates;
+    const double cap =
+        s->batch_cap_ms > 0 ? std::max(s->batch_cap_ms * rate_per_ms(s), 0.25 * queued) : 0.0;
+    double work = 0;
     while (!s->queue.empty() && (int)batch.size() < s->max_batch) {
         Job* j = s->queue.front();
+        if (!batch.empty() && cap > 0 && work + j->updates > cap) break;
         size_t oo = 0, fo = 0;
         const size_t need_bytes = Pool::rounded(j->out_bytes) + Pool::rounded(j->filt_bytes);
         if (need_bytes > s->pool.size()) {  // can never fit
@@ -165,6 +211,7 @@ bool launch_into(ct_hip_server* s, int k, std::unique_lock<std::mutex>& lk) {
         j->slot = k;
         j->t_admit = now_ms();
         batch.push_back(j);
+        work += j->updates;
     }
     if (batch.empty()) return false;
     std::vector<ct_fbp_job> desc(batch.size());
@@ -198,6 +245,7 @@ bool launch_into(ct_hip_server* s, int k, std::unique_lock<std::mutex>& lk) {
         return true;
     }
     sl.jobs = std::move(batch);
+    if (inflight_slots == 0) s->busy_start = now_ms();
     sl.busy = true;
     s->batches++;
     if (inflight_slots > 0) s->overlapped_launches++;
@@ -225,6 +273,7 @@ void worker_loop(ct_hip_server* s) {
                     j->t_done = t;
                     s->done.push_back(j);
                     s->completed++;
+                    s->done_updates += j->updates;
                 } else {
                     s->pool.release(j->out_off);
                     fail_job(s, j, std::string("batch failed: ") + hipGetErrorString(q));
@@ -234,6 +283,12 @@ void worker_loop(ct_hip_server* s) {
             sl.jobs.clear();
             sl.busy = false;
             progress = true;
+            bool any = false;
+            for (auto& o : s->slots) any |= o.busy;
+            if (!any && s->busy_start >= 0) {
+                s->busy_ms += t - s->busy_start;
+                s->busy_start = -1;
+            }
         }
         if (progress) s->cv_done.notify_all();
         for (int k = 0; k < (int)s->slots.size(); ++k)
@@ -314,6 +369,17 @@ int ct_hip_server_set_flags(ct_hip_server* s, int flags) {
     return CT_OK;
 }
 
+int ct_hip_server_set_policy(ct_hip_server* s, int policy, double batch_cap_ms,
+                             double age_weight) {
+    if (!s || (policy != CT_SERVE_FIFO && policy != CT_SERVE_SIZE_AGED)) return CT_ERR_INVALID;
+    if (!(batch_cap_ms >= 0) || !(age_weight >= 0)) return CT_ERR_INVALID;
+    std::lock_guard<std::mutex> g(s->mu);
+    s->policy = policy;
+    s->batch_cap_ms = batch_cap_ms;
+    s->age_weight = age_weight;
+    return CT_OK;
+}
+
 int ct_hip_server_submit(ct_hip_server* s, const ct_fbp_job* job, uint64_t* ticket) {
     if (!s || !job || !ticket || !job->d_sinos || !job->angles_deg) return CT_ERR_INVALID;
     if (job->nz <= 0 || job->na <= 0 || job->bins <= 0) return CT_ERR_INVALID;
@@ -325,6 +391,7 @@ int ct_hip_server_submit(ct_hip_server* s, const ct_fbp_job* job, uint64_t* tick
     j->angles.assign(job->angles_deg, job->angles_deg + job->na);
     j->out_bytes = sizeof(float) * (size_t)job->nz * job->bins * job->bins;
     j->filt_bytes = job_filt_bytes(*job);
+    j->updates = (double)job->nz * job->bins * (double)job->bins * job->na;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         j->ticket = s->next_ticket++;

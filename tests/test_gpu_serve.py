@@ -124,3 +124,71 @@ def test_oversize_job_fails_cleanly(inputs):
     assert srv2.wait(30000)[0] == t2
     srv.close()
     srv2.close()
+
+
+def test_policies_give_identical_results(inputs):
+    ctx, jobs = inputs
+    outs = {}
+    for policy, cap in ((ct.SERVE_FIFO, 0.0), (ct.SERVE_SIZE_AGED, 0.05), (ct.SERVE_SIZE_AGED, 0.0)):
+        srv = ct.Server(pool_bytes=256 << 20, max_batch=8, slots=2)
+        srv.set_policy(policy, batch_cap_ms=cap)
+        tick = {srv.submit(j["dev"], j["ang"], j["nz"], j["na"], j["N"], j["f"]): k
+                for k, j in enumerate(jobs)}
+        for _ in jobs:
+            r = srv.wait(30000)
+            assert r is not None, "server stalled"
+            k = tick[r[0]]
+            j = jobs[k]
+            out = srv.download(r[0], (j["nz"], j["N"], j["N"]))
+            srv.release(r[0])
+            if k in outs:
+                assert np.array_equal(out.view(np.uint32), outs[k].view(np.uint32)), (policy, k)
+            else:
+                outs[k] = out
+        srv.close()
+
+
+def test_set_policy_rejects_bad_arguments():
+    srv = ct.Server(pool_bytes=1 << 20, max_batch=2, slots=1)
+    for args in ((7, 5.0, 0.25), (ct.SERVE_FIFO, -1.0, 0.25), (ct.SERVE_SIZE_AGED, 5.0, -0.5)):
+        with pytest.raises(ct.BackendError):
+            srv.set_policy(*args)
+    srv.close()
+
+
+def test_size_order_lets_small_jobs_pass_a_large_one():
+    """One batch per job, one slot: while a long job runs, a second long job
+    and three small ones queue. FIFO runs them in submission order; the
+    size-ordered policy runs the small ones before the queued long one."""
+    ctx = ct.Context(0)
+    N, nz, na = 768, 512, 720  # ~2e11 voxel-updates: tens of ms on the device
+    big = ctx.alloc(4 * nz * na * N).upload(np.zeros((nz, na, N), np.float32))
+    n, snz, sna = 64, 4, 60
+    small = ctx.alloc(4 * snz * sna * n).upload(np.ones((snz, sna, n), np.float32))
+    ang_big = (np.float32(180.0 / na) * np.arange(na, dtype=np.float32)).astype(np.float32)
+    ang_small = (np.float32(180.0 / sna) * np.arange(sna, dtype=np.float32)).astype(np.float32)
+    orders = {}
+    for policy in (ct.SERVE_FIFO, ct.SERVE_SIZE_AGED):
+        srv = ct.Server(pool_bytes=8 << 30, max_batch=1, slots=1)
+        srv.set_policy(policy, batch_cap_ms=5.0, age_weight=0.25)
+        name = {srv.submit(big, ang_big, nz, na, N, 0): "B1"}
+        t0 = time.time()
+        while srv.stats()["batches"] < 1:  # B1 is admitted alone and runs
+            assert time.time() - t0 < 30, "B1 not admitted"
+            time.sleep(0.0005)
+        name[srv.submit(big, ang_big, nz, na, N, 0)] = "B2"
+        for k in range(3):
+            name[srv.submit(small, ang_small, snz, sna, n, 0)] = f"S{k}"
+        seq = []
+        for _ in range(5):
+            r = srv.wait(60000)
+            assert r is not None, "server stalled"
+            seq.append(name[r[0]])
+            srv.release(r[0])
+        srv.close()
+        orders[policy] = seq
+    assert orders[ct.SERVE_FIFO] == ["B1", "B2", "S0", "S1", "S2"]
+    assert orders[ct.SERVE_SIZE_AGED] == ["B1", "S0", "S1", "S2", "B2"]
+    big.free()
+    small.free()
+    ctx.close()
