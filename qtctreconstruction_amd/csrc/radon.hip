@@ -42,16 +42,30 @@ __global__ void minmax_init_kernel(unsigned* keys, int nz) {
 }
 
 // grid (chunks, nz): block-reduce a chunk of one slice, then one atomic each.
+// vec4: every slice starts 16-byte aligned and holds a multiple of 4 values,
+// so the slice is read as float4.
 __global__ __launch_bounds__(256) void minmax_kernel(const float* __restrict__ vol,
-                                                     size_t slice_elems, unsigned* keys) {
+                                                     size_t slice_elems, int vec4,
+                                                     unsigned* keys) {
     const int z = blockIdx.y;
     const float* p = vol + (size_t)z * slice_elems;
     unsigned kmin = 0xffffffffu, kmax = 0u;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < slice_elems; i += stride) {
-        const unsigned k = f2key(p[i]);
-        kmin = min(kmin, k);
-        kmax = max(kmax, k);
+    const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (vec4) {
+        const float4* p4 = reinterpret_cast<const float4*>(p);
+        for (size_t i = first; i < slice_elems / 4; i += stride) {
+            const float4 q = p4[i];
+            const unsigned a = f2key(q.x), b = f2key(q.y), c = f2key(q.z), d = f2key(q.w);
+            kmin = min(kmin, min(min(a, b), min(c, d)));
+            kmax = max(kmax, max(max(a, b), max(c, d)));
+        }
+    } else {
+        for (size_t i = first; i < slice_elems; i += stride) {
+            const unsigned k = f2key(p[i]);
+            kmin = min(kmin, k);
+            kmax = max(kmax, k);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         kmin = min(kmin, (unsigned)__shfl_xor((int)kmin, off));
@@ -79,24 +93,32 @@ __global__ void minmax_decode_kernel(const unsigned* keys, int nz, float* mm) {
     if (i < 2 * nz) mm[i] = key2f(keys[i]);
 }
 
-// block (32,8) handles a 32x32 pixel tile of one Radon z-block (kRZ = 8
-// slices): reads the slices, normalises (SerialBackend.cpp:78-80,104,106),
-// writes the row copy directly and the column copy (the transpose) through
-// an LDS transpose. A pixel is 32 bytes (the 8 slices as 2 float4):
+// One block normalises a tile of kNormTW x kNormTH pixels of one Radon
+// z-block (kRZ = 8 slices; SerialBackend.cpp:78-80,104,106) and writes it to
+// both copies. A pixel is 32 bytes (the 8 slices as 2 float4):
 //   row copy     byte ((zb*h + y)*(w + kPadX) + x)*32
 //   column copy  byte ((zb*w + x)*(h + kPadX) + y)*32
 // The kPadX pixels past each row of either copy (x >= w in the row copy,
 // y >= h in the column copy) are zeros, so a lane's fixed-size chunk of
 // candidates may run past the edge of the image and read only zeros.
+// Each thread reads 4 pixels of one row from each slice (float4 loads when
+// vec4: w % 4 == 0 and a 16-byte aligned volume) and puts them in an LDS
+// tile; both copies are then written from the tile with consecutive lanes
+// on consecutive 16-byte units: 1 KB contiguous per wave store in the row
+// copy, a column's 512 bytes per 32 lanes in the column copy. Tile rows are padded by one
+// float4, so the column reads (16 rows apart) spread over the banks.
+constexpr int kNormTW = 64, kNormTH = 16;
+constexpr int kNormRow = kNormTW * 2 + 1;  // float4 per padded tile row
 __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict__ vol, int w,
                                                         int h, int nz,
                                                         const float* __restrict__ mm,
-                                                        float4* __restrict__ rowL,
+                                                        int vec4, float4* __restrict__ rowL,
                                                         float4* __restrict__ colL) {
-    __shared__ float4 tile[32][33][2];
+    __shared__ float4 tile[kNormTH * kNormRow];
     const int zb = blockIdx.z;
-    const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+    const int bx = blockIdx.x * kNormTW, by = blockIdx.y * kNormTH;
     const int wp = w + kPadX, hp = h + kPadX;
+    const int t = threadIdx.x;
     float mn[kRZ], inv[kRZ];
     bool live[kRZ];
 #pragma unroll
@@ -111,40 +133,69 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
         inv[k] = 1.0f / span;
     }
     const size_t ss = (size_t)w * h;
-    float4* __restrict__ r0 = rowL + (size_t)zb * h * wp * 2;
-    float4* __restrict__ c0 = colL + (size_t)zb * w * hp * 2;
-    for (int r = threadIdx.y; r < 32; r += 8) {
-        const int x = bx + threadIdx.x, y = by + r;
-        float t[kRZ];
+    {
+        const int tx = t & 15, ty = t >> 4;
+        const int x = bx + 4 * tx, y = by + ty;
+        float v[kRZ][4];
 #pragma unroll
-        for (int k = 0; k < kRZ; ++k) t[k] = 0.0f;
-        if (x < w && y < h) {
+        for (int k = 0; k < kRZ; ++k)
 #pragma unroll
-            for (int k = 0; k < kRZ; ++k) {
-                if (live[k]) {
-                    const float raw = vol[(size_t)(zb * kRZ + k) * ss + (size_t)y * w + x];
-                    t[k] = (raw == 0.0f) ? 0.0f : (raw - mn[k]) * inv[k];
-                }
+            for (int j = 0; j < 4; ++j) v[k][j] = 0.0f;
+        if (y < h && x < w) {
+            const float* row = vol + (size_t)zb * kRZ * ss + (size_t)y * w + x;
+            if (vec4) {  // x and every row start are 16-byte aligned
+#pragma unroll
+                for (int k = 0; k < kRZ; ++k)
+                    if (live[k]) {
+                        const float4 q = *reinterpret_cast<const float4*>(row + k * ss);
+                        v[k][0] = q.x;
+                        v[k][1] = q.y;
+                        v[k][2] = q.z;
+                        v[k][3] = q.w;
+                    }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kRZ; ++k)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        if (live[k] && x + j < w) v[k][j] = row[k * ss + j];
             }
         }
-        const float4 v0 = make_float4(t[0], t[1], t[2], t[3]);
-        const float4 v1 = make_float4(t[4], t[5], t[6], t[7]);
-        if (x < wp && y < h) {  // pixels w..w+kPadX-1 are the zero pad
-            float4* d = r0 + ((size_t)y * wp + x) * 2;
-            d[0] = v0;
-            d[1] = v1;
+#pragma unroll
+        for (int k = 0; k < kRZ; ++k)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float raw = v[k][j];
+                v[k][j] = (raw == 0.0f) ? 0.0f : (raw - mn[k]) * inv[k];
+            }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float4* d = tile + ty * kNormRow + (4 * tx + j) * 2;
+            d[0] = make_float4(v[0][j], v[1][j], v[2][j], v[3][j]);
+            d[1] = make_float4(v[4][j], v[5][j], v[6][j], v[7][j]);
         }
-        tile[r][threadIdx.x][0] = v0;
-        tile[r][threadIdx.x][1] = v1;
     }
     __syncthreads();
-    for (int r = threadIdx.y; r < 32; r += 8) {
-        const int y = by + threadIdx.x, x = bx + r;
-        if (x < w && y < hp) {  // pixels h..h+kPadX-1 of a column are the zero pad
-            float4* d = c0 + ((size_t)x * hp + y) * 2;
-            d[0] = tile[threadIdx.x][r][0];
-            d[1] = tile[threadIdx.x][r][1];
-        }
+    // row copy: tile row r is 2*kNormTW float4 of one copy row (pixels w..
+    // w+kPadX-1 are the zero pad; rows y >= h are not part of this copy)
+    float4* __restrict__ r0 = rowL + (size_t)zb * h * wp * 2;
+#pragma unroll
+    for (int i = 0; i < kNormTH * kNormTW * 2 / 256; ++i) {
+        const int g = i * 256 + t;
+        const int r = g / (2 * kNormTW), f = g - r * (2 * kNormTW);
+        const int y = by + r, x = bx + (f >> 1);
+        if (y < h && x < wp) r0[((size_t)y * wp + x) * 2 + (f & 1)] = tile[r * kNormRow + f];
+    }
+    // column copy: column c is kNormTH pixels = 2*kNormTH float4 (rows
+    // h..h+kPadX-1 are the zero pad; columns x >= w are not part of it)
+    float4* __restrict__ c0 = colL + (size_t)zb * w * hp * 2;
+#pragma unroll
+    for (int i = 0; i < kNormTH * kNormTW * 2 / 256; ++i) {
+        const int g = i * 256 + t;
+        const int c = g / (2 * kNormTH), q = g - c * (2 * kNormTH);
+        const int x = bx + c, y = by + (q >> 1);
+        if (x < w && y < hp)
+            c0[((size_t)x * hp + y) * 2 + (q & 1)] = tile[(q >> 1) * kNormRow + c * 2 + (q & 1)];
     }
 }
 
@@ -398,9 +449,11 @@ hipError_t launch_minmax_init(unsigned* d_keys, int nz, hipStream_t s) {
 
 hipError_t launch_minmax(const float* d_vol, size_t slice_elems, int nz, unsigned* d_keys,
                          hipStream_t s) {
-    int chunks = (int)((slice_elems + 256 * 16 - 1) / (256 * 16));
+    // 64 KB of a slice per block (16 float4 per thread)
+    int chunks = (int)((slice_elems + 256 * 64 - 1) / (256 * 64));
     chunks = chunks < 1 ? 1 : (chunks > 64 ? 64 : chunks);
-    hipLaunchKernelGGL(minmax_kernel, dim3(chunks, nz), dim3(256), 0, s, d_vol, slice_elems,
+    const int vec4 = (slice_elems % 4 == 0) && ((uintptr_t)d_vol % 16 == 0);
+    hipLaunchKernelGGL(minmax_kernel, dim3(chunks, nz), dim3(256), 0, s, d_vol, slice_elems, vec4,
                        d_keys);
     return hipGetLastError();
 }
@@ -414,9 +467,10 @@ hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hip
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, hipStream_t s) {
     const int nzb = (nz + kRZ - 1) / kRZ;
-    dim3 grid((w + kPadX + 31) / 32, (h + kPadX + 31) / 32, nzb);
-    hipLaunchKernelGGL(normalise_kernel, grid, dim3(32, 8), 0, s, d_vol, w, h, nz, d_mm, d_row,
-                       d_col);
+    dim3 grid((w + kPadX + kNormTW - 1) / kNormTW, (h + kPadX + kNormTH - 1) / kNormTH, nzb);
+    const int vec4 = (w % 4 == 0) && ((uintptr_t)d_vol % 16 == 0);
+    hipLaunchKernelGGL(normalise_kernel, grid, dim3(256), 0, s, d_vol, w, h, nz, d_mm, vec4,
+                       d_row, d_col);
     return hipGetLastError();
 }
 
