@@ -128,11 +128,12 @@ struct FiltJob {
 FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, int pad, int nfft,
                         const float* d_H, const float2* d_tw, float4* d_filt);
 size_t fft_lds_bytes(int nfft);
-int filter_persistent_grid(int max_nfft, int device);
+// One persistent launch per FFT length: every job in its list has nfft.
+int filter_persistent_grid(int nfft, int device);
 // hdr = {first job index in d_jobs, njobs, total items}; counter must be 0
 // at launch. Everything batch-dependent is read from device memory.
 hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
-                                        int max_nfft, int grid, hipStream_t s);
+                                        int nfft, int grid, hipStream_t s);
 // FilterDesign::createFilter(n, type): n/2 + 1 taps, the reference's FP32
 // arithmetic bit for bit (host; n a power of two >= 4)
 void serial_filter_taps(int n, int type, float* H);

@@ -41,11 +41,13 @@ __global__ void twiddle_kernel(int n, float2* __restrict__ tw) {
     tw[idx] = make_float2((float)cos(a), (float)sin(a));
 }
 
+// Complex products with one rounded product and one FMA per component
+// (4 VALU instead of 6).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
-    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+    return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
 }
 
 // Global-memory views of the argument pointers: the persistent kernel reads
@@ -66,7 +68,7 @@ typedef f4v __attribute__((address_space(1))) g_float4;
 // 4096 is 1.07 (from 4.1).
 __device__ __forceinline__ int swz(int e) { return e ^ ((e >> 3) & 31); }
 
-// Radix-2 butterfly stages on the two LDS transforms X0, X1 of n points.
+// Radix-2 butterfly stages on the two LDS transforms X0, X1 of N points.
 // Stage of half-span m, element e at position q = e mod 2m < m paired with
 // e + m; W = W_2m^q (stage order: W_M^j at M/2 - 1 + j), conjugated when
 // CONJ (the inverse transform):
@@ -74,188 +76,212 @@ __device__ __forceinline__ int swz(int e) { return e ^ ((e >> 3) & 31); }
 //       t = W*b;  a' = a + t;  b' = a - t
 //   DIF (decreasing spans, natural in, bit-reversed out):
 //       a' = a + b;  b' = W*(a - b)
-// A fused pass applies 1, 2 or 3 consecutive stages to groups of 2, 4 or 8
-// elements in registers, then writes them back; every stage's operations
-// and their order are those of the single stage.
+// A fused pass applies 3 consecutive stages (half-spans H, 2H, 4H) to groups
+// of 8 elements in registers, then writes them back; every stage's
+// operations and their order are those of the single stage. N and H are
+// compile-time constants, so the group and LDS-slot arithmetic folds into
+// shifts, masks and immediate offsets. When log2 N is not a multiple of 3,
+// the 1 or 2 innermost stages run on groups of 2 or 4 at half-spans 1 and 2,
+// whose twiddles are exactly 1 and -i (applied as such).
 template <bool CONJ>
 __device__ __forceinline__ float2 twmul(float2 a, float2 w) {
     return CONJ ? cmulc(a, w) : cmul(a, w);
 }
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// a * (-i), or a * conj(-i) = a * i for the inverse
+template <bool CONJ>
+__device__ __forceinline__ float2 mul_mi(float2 a) {
+    return CONJ ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
 
-template <bool CONJ, class TW>
-__device__ __forceinline__ void fft_dit8(float2* X0, float2* X1, int n, int h, const TW& T,
-                                         int tid) {
-    for (int j = tid; j < n / 8; j += 256) {
-        const int g = j / h, k = j - g * h;
-        const int base = g * 8 * h + k;
-        const float2 w1 = T(h - 1 + k);
-        const float2 w2a = T(2 * h - 1 + k), w2b = T(3 * h - 1 + k);
+// The three stages of a fused pass on 8 elements in registers: DIT at
+// half-spans H, 2H, 4H (w1: W_2H^k, w2a/w2b: W_4H^k, W_4H^(k+H), w4[r]:
+// W_8H^(k+rH)) and DIF in the reverse order.
+template <bool CONJ>
+__device__ __forceinline__ void dit8_regs(float2 (&x)[8], float2 w1, float2 w2a, float2 w2b,
+                                          const float2 (&w4)[4]) {
+#pragma unroll
+    for (int r = 0; r < 8; r += 2) {  // half-span H
+        const float2 t = twmul<CONJ>(x[r + 1], w1);
+        x[r + 1] = csub(x[r], t);
+        x[r] = cadd(x[r], t);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r += 4) {  // half-span 2H
+        const float2 t0 = twmul<CONJ>(x[r + 2], w2a);
+        x[r + 2] = csub(x[r], t0);
+        x[r] = cadd(x[r], t0);
+        const float2 t1 = twmul<CONJ>(x[r + 3], w2b);
+        x[r + 3] = csub(x[r + 1], t1);
+        x[r + 1] = cadd(x[r + 1], t1);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {  // half-span 4H
+        const float2 t = twmul<CONJ>(x[r + 4], w4[r]);
+        x[r + 4] = csub(x[r], t);
+        x[r] = cadd(x[r], t);
+    }
+}
+template <bool CONJ>
+__device__ __forceinline__ void dif8_regs(float2 (&x)[8], float2 w1, float2 w2a, float2 w2b,
+                                          const float2 (&w4)[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {  // half-span 4H
+        const float2 d = csub(x[r], x[r + 4]);
+        x[r] = cadd(x[r], x[r + 4]);
+        x[r + 4] = twmul<CONJ>(d, w4[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r += 4) {  // half-span 2H
+        const float2 d0 = csub(x[r], x[r + 2]);
+        x[r] = cadd(x[r], x[r + 2]);
+        x[r + 2] = twmul<CONJ>(d0, w2a);
+        const float2 d1 = csub(x[r + 1], x[r + 3]);
+        x[r + 1] = cadd(x[r + 1], x[r + 3]);
+        x[r + 3] = twmul<CONJ>(d1, w2b);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r += 2) {  // half-span H
+        const float2 d = csub(x[r], x[r + 1]);
+        x[r] = cadd(x[r], x[r + 1]);
+        x[r + 1] = twmul<CONJ>(d, w1);
+    }
+}
+// Half-spans 1 and 2 on a group of 4 (twiddles W_2^0 = 1, W_4^0 = 1,
+// W_4^1 = -i): DIT increasing, DIF decreasing.
+template <bool CONJ>
+__device__ __forceinline__ void dit4_regs(float2 (&x)[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; r += 2) {
+        const float2 t = x[r + 1];
+        x[r + 1] = csub(x[r], t);
+        x[r] = cadd(x[r], t);
+    }
+    const float2 t0 = x[2];
+    x[2] = csub(x[0], t0);
+    x[0] = cadd(x[0], t0);
+    const float2 t1 = mul_mi<CONJ>(x[3]);
+    x[3] = csub(x[1], t1);
+    x[1] = cadd(x[1], t1);
+}
+template <bool CONJ>
+__device__ __forceinline__ void dif4_regs(float2 (&x)[4]) {
+    const float2 d0 = csub(x[0], x[2]);
+    x[0] = cadd(x[0], x[2]);
+    x[2] = d0;
+    const float2 d1 = csub(x[1], x[3]);
+    x[1] = cadd(x[1], x[3]);
+    x[3] = mul_mi<CONJ>(d1);
+#pragma unroll
+    for (int r = 0; r < 4; r += 2) {
+        const float2 d = csub(x[r], x[r + 1]);
+        x[r] = cadd(x[r], x[r + 1]);
+        x[r + 1] = d;
+    }
+}
+
+// One fused pass of 8-element groups base + r*H over both transforms.
+template <bool DIT, bool CONJ, int N, int H, class TW>
+__device__ __forceinline__ void fft_pass8(float2* X0, float2* X1, const TW& T, int tid) {
+    for (int j = tid; j < N / 8; j += 256) {
+        const int g = j / H, k = j - g * H;
+        const int base = g * 8 * H + k;
+        const float2 w1 = T(H - 1 + k);
+        const float2 w2a = T(2 * H - 1 + k), w2b = T(3 * H - 1 + k);
         float2 w4[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) w4[r] = T(4 * h - 1 + k + r * h);
+        for (int r = 0; r < 4; ++r) w4[r] = T(4 * H - 1 + k + r * H);
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             float2* X = f ? X1 : X0;
             float2 x[8];
 #pragma unroll
-            for (int r = 0; r < 8; ++r) x[r] = X[swz(base + r * h)];
+            for (int r = 0; r < 8; ++r) x[r] = X[swz(base + r * H)];
+            if constexpr (DIT) dit8_regs<CONJ>(x, w1, w2a, w2b, w4);
+            else dif8_regs<CONJ>(x, w1, w2a, w2b, w4);
 #pragma unroll
-            for (int r = 0; r < 8; r += 2) {  // half-span h
-                const float2 t = twmul<CONJ>(x[r + 1], w1);
-                x[r + 1] = make_float2(x[r].x - t.x, x[r].y - t.y);
-                x[r] = make_float2(x[r].x + t.x, x[r].y + t.y);
-            }
-#pragma unroll
-            for (int r = 0; r < 8; r += 4) {  // half-span 2h
-                const float2 t0 = twmul<CONJ>(x[r + 2], w2a);
-                x[r + 2] = make_float2(x[r].x - t0.x, x[r].y - t0.y);
-                x[r] = make_float2(x[r].x + t0.x, x[r].y + t0.y);
-                const float2 t1 = twmul<CONJ>(x[r + 3], w2b);
-                x[r + 3] = make_float2(x[r + 1].x - t1.x, x[r + 1].y - t1.y);
-                x[r + 1] = make_float2(x[r + 1].x + t1.x, x[r + 1].y + t1.y);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {  // half-span 4h
-                const float2 t = twmul<CONJ>(x[r + 4], w4[r]);
-                x[r + 4] = make_float2(x[r].x - t.x, x[r].y - t.y);
-                x[r] = make_float2(x[r].x + t.x, x[r].y + t.y);
-            }
-#pragma unroll
-            for (int r = 0; r < 8; ++r) X[swz(base + r * h)] = x[r];
+            for (int r = 0; r < 8; ++r) X[swz(base + r * H)] = x[r];
         }
     }
     __syncthreads();
 }
 
-template <bool CONJ, class TW>
-__device__ __forceinline__ void fft_dif8(float2* X0, float2* X1, int n, int h, const TW& T,
-                                         int tid) {
-    for (int j = tid; j < n / 8; j += 256) {
-        const int g = j / h, k = j - g * h;
-        const int base = g * 8 * h + k;
-        const float2 w1 = T(h - 1 + k);
-        const float2 w2a = T(2 * h - 1 + k), w2b = T(3 * h - 1 + k);
-        float2 w4[4];
+// The middle of the filter in one LDS round trip: the forward transform's
+// innermost stages (half-spans < G, on groups of G consecutive elements),
+// the product with the filter (element q holds frequency brev(q)), and the
+// inverse transform's first stages on the same groups.
+template <int LOGN, int G, class TW>
+__device__ __forceinline__ void fft_middle(float2* X0, float2* X1, const TW& T,
+                                           g_cfloat* __restrict__ H, int tid) {
+    constexpr int N = 1 << LOGN;
+    float2 w1 = make_float2(1.0f, 0.0f), w2a = w1, w2b = w1, w4[4] = {w1, w1, w1, w1};
+    if constexpr (G == 8) {  // half-span 1 twiddles of a fused radix-8 pass
+        w1 = T(0);
+        w2a = T(1);
+        w2b = T(2);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) w4[r] = T(4 * h - 1 + k + r * h);
+        for (int r = 0; r < 4; ++r) w4[r] = T(3 + r);
+    }
+    for (int j = tid; j < N / G; j += 256) {
+        float hv[G];
+#pragma unroll
+        for (int r = 0; r < G; ++r) {
+            const int k = (int)(__brev((unsigned)(G * j + r)) >> (32 - LOGN));
+            hv[r] = H[k <= N / 2 ? k : N - k];
+        }
 #pragma unroll
         for (int f = 0; f < 2; ++f) {
             float2* X = f ? X1 : X0;
-            float2 x[8];
+            float2 x[G];
 #pragma unroll
-            for (int r = 0; r < 8; ++r) x[r] = X[swz(base + r * h)];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {  // half-span 4h
-                const float2 d = make_float2(x[r].x - x[r + 4].x, x[r].y - x[r + 4].y);
-                x[r] = make_float2(x[r].x + x[r + 4].x, x[r].y + x[r + 4].y);
-                x[r + 4] = twmul<CONJ>(d, w4[r]);
+            for (int r = 0; r < G; ++r) x[r] = X[swz(G * j + r)];
+            if constexpr (G == 8) dif8_regs<false>(x, w1, w2a, w2b, w4);
+            else if constexpr (G == 4) dif4_regs<false>(x);
+            else {
+                const float2 d = csub(x[0], x[1]);
+                x[0] = cadd(x[0], x[1]);
+                x[1] = d;
             }
 #pragma unroll
-            for (int r = 0; r < 8; r += 4) {  // half-span 2h
-                const float2 d0 = make_float2(x[r].x - x[r + 2].x, x[r].y - x[r + 2].y);
-                x[r] = make_float2(x[r].x + x[r + 2].x, x[r].y + x[r + 2].y);
-                x[r + 2] = twmul<CONJ>(d0, w2a);
-                const float2 d1 = make_float2(x[r + 1].x - x[r + 3].x, x[r + 1].y - x[r + 3].y);
-                x[r + 1] = make_float2(x[r + 1].x + x[r + 3].x, x[r + 1].y + x[r + 3].y);
-                x[r + 3] = twmul<CONJ>(d1, w2b);
+            for (int r = 0; r < G; ++r) x[r] = make_float2(x[r].x * hv[r], x[r].y * hv[r]);
+            if constexpr (G == 8) dit8_regs<true>(x, w1, w2a, w2b, w4);
+            else if constexpr (G == 4) dit4_regs<true>(x);
+            else {
+                const float2 d = csub(x[0], x[1]);
+                x[0] = cadd(x[0], x[1]);
+                x[1] = d;
             }
 #pragma unroll
-            for (int r = 0; r < 8; r += 2) {  // half-span h
-                const float2 d = make_float2(x[r].x - x[r + 1].x, x[r].y - x[r + 1].y);
-                x[r] = make_float2(x[r].x + x[r + 1].x, x[r].y + x[r + 1].y);
-                x[r + 1] = twmul<CONJ>(d, w1);
-            }
-#pragma unroll
-            for (int r = 0; r < 8; ++r) X[swz(base + r * h)] = x[r];
+            for (int r = 0; r < G; ++r) X[swz(G * j + r)] = x[r];
         }
     }
     __syncthreads();
 }
 
-template <bool CONJ, class TW>
-__device__ __forceinline__ void fft_dit4(float2* X0, float2* X1, int n, int h, const TW& T,
-                                         int tid) {
-    for (int j = tid; j < n / 4; j += 256) {
-        const int g = j / h, k = j - g * h;
-        const int base = g * 4 * h + k;
-        const float2 w1 = T(h - 1 + k), w2a = T(2 * h - 1 + k), w2b = T(3 * h - 1 + k);
-#pragma unroll
-        for (int f = 0; f < 2; ++f) {
-            float2* X = f ? X1 : X0;
-            float2 x[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) x[r] = X[swz(base + r * h)];
-#pragma unroll
-            for (int r = 0; r < 4; r += 2) {
-                const float2 t = twmul<CONJ>(x[r + 1], w1);
-                x[r + 1] = make_float2(x[r].x - t.x, x[r].y - t.y);
-                x[r] = make_float2(x[r].x + t.x, x[r].y + t.y);
-            }
-            const float2 t0 = twmul<CONJ>(x[2], w2a);
-            x[2] = make_float2(x[0].x - t0.x, x[0].y - t0.y);
-            x[0] = make_float2(x[0].x + t0.x, x[0].y + t0.y);
-            const float2 t1 = twmul<CONJ>(x[3], w2b);
-            x[3] = make_float2(x[1].x - t1.x, x[1].y - t1.y);
-            x[1] = make_float2(x[1].x + t1.x, x[1].y + t1.y);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) X[swz(base + r * h)] = x[r];
-        }
+// Fused DIF passes at half-spans H, H/8, ... >= LAST, and DIT passes at
+// H, 8H, ... < N.
+template <int N, int H, int LAST, class TW>
+__device__ __forceinline__ void dif8_passes(float2* X0, float2* X1, const TW& T, int tid) {
+    if constexpr (H >= LAST) {
+        fft_pass8<false, false, N, H>(X0, X1, T, tid);
+        dif8_passes<N, H / 8, LAST>(X0, X1, T, tid);
     }
-    __syncthreads();
+}
+template <int N, int H, class TW>
+__device__ __forceinline__ void dit8_passes(float2* X0, float2* X1, const TW& T, int tid) {
+    if constexpr (H < N) {
+        fft_pass8<true, true, N, H>(X0, X1, T, tid);
+        dit8_passes<N, H * 8>(X0, X1, T, tid);
+    }
 }
 
-template <bool CONJ, class TW>
-__device__ __forceinline__ void fft_dif4(float2* X0, float2* X1, int n, int h, const TW& T,
-                                         int tid) {
-    for (int j = tid; j < n / 4; j += 256) {
-        const int g = j / h, k = j - g * h;
-        const int base = g * 4 * h + k;
-        const float2 w1 = T(h - 1 + k), w2a = T(2 * h - 1 + k), w2b = T(3 * h - 1 + k);
-#pragma unroll
-        for (int f = 0; f < 2; ++f) {
-            float2* X = f ? X1 : X0;
-            float2 x[4];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) x[r] = X[swz(base + r * h)];
-            const float2 d0 = make_float2(x[0].x - x[2].x, x[0].y - x[2].y);
-            x[0] = make_float2(x[0].x + x[2].x, x[0].y + x[2].y);
-            x[2] = twmul<CONJ>(d0, w2a);
-            const float2 d1 = make_float2(x[1].x - x[3].x, x[1].y - x[3].y);
-            x[1] = make_float2(x[1].x + x[3].x, x[1].y + x[3].y);
-            x[3] = twmul<CONJ>(d1, w2b);
-#pragma unroll
-            for (int r = 0; r < 4; r += 2) {
-                const float2 d = make_float2(x[r].x - x[r + 1].x, x[r].y - x[r + 1].y);
-                x[r] = make_float2(x[r].x + x[r + 1].x, x[r].y + x[r + 1].y);
-                x[r + 1] = twmul<CONJ>(d, w1);
-            }
-#pragma unroll
-            for (int r = 0; r < 4; ++r) X[swz(base + r * h)] = x[r];
-        }
-    }
-    __syncthreads();
-}
-
-// half-span 1 (twiddle 1)
-__device__ __forceinline__ void fft_dit2(float2* X0, float2* X1, int n, int tid) {
-    for (int b = tid; b < n / 2; b += 256) {
-        const int i0 = swz(2 * b), i1 = swz(2 * b + 1);
-        const float2 u0 = X0[i0], v0 = X0[i1], u1 = X1[i0], v1 = X1[i1];
-        X0[i0] = make_float2(u0.x + v0.x, u0.y + v0.y);
-        X0[i1] = make_float2(u0.x - v0.x, u0.y - v0.y);
-        X1[i0] = make_float2(u1.x + v1.x, u1.y + v1.y);
-        X1[i1] = make_float2(u1.x - v1.x, u1.y - v1.y);
-    }
-    __syncthreads();
-}
-__device__ __forceinline__ void fft_dif2(float2* X0, float2* X1, int n, int tid) {
-    fft_dit2(X0, X1, n, tid);  // with twiddle 1 both forms are a + b, a - b
-}
-
-// One (angle, z-block) work item: 4 rows -> 2 complex FFTs of n points in LDS.
+// One (angle, z-block) work item: 4 rows -> 2 complex FFTs of n = 2^LOGN
+// points in LDS.
+template <int LOGN>
 __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb,
                                                 float2* __restrict__ sm) {
-    const int n = p.nfft, logn = p.logn;
+    constexpr int n = 1 << LOGN, logn = LOGN;
     float2* X0 = sm;
     float2* X1 = sm + n;
     g_cfloat2* __restrict__ Tg = (g_cfloat2*)p.tw;  // stage-ordered, L1-resident
@@ -265,7 +291,6 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     };
     g_cfloat* __restrict__ H = (g_cfloat*)p.H;
     const int tid = threadIdx.x;
-    const int nh = n / 2;
     g_cfloat* rows[kZB];
 #pragma unroll
     for (int r = 0; r < kZB; ++r) {
@@ -288,33 +313,16 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     // No element is ever moved to its bit-reversed place. Radix-2 stages are
     // fused three at a time (a thread takes the 8 elements base + r*h through
     // the stages of half-span 4h, 2h, h, or h, 2h, 4h, in registers: the same
-    // operations as the separate stages with one LDS round trip); the 1 or 2
-    // remaining stages run as one radix-2 or radix-4 pass.
-    const int rem = logn % 3;
-    int h = n / 8;
-    for (int pass = 0; pass < logn / 3; ++pass, h >>= 3)
-        fft_dif8<false>(X0, X1, n, h, T, tid);
-    if (rem == 1) fft_dif2(X0, X1, n, tid);
-    else if (rem == 2) fft_dif4<false>(X0, X1, n, 1, T, tid);
-    // multiply by the real, even filter (H[n-k] = H[k]); element p holds
-    // frequency brev(p)
-    for (int q = tid; q < n; q += 256) {
-        const int k = (int)(__brev((unsigned)q) >> (32 - logn));
-        const float hv = H[k <= nh ? k : n - k];
-        const int qs = swz(q);
-        X0[qs] = make_float2(X0[qs].x * hv, X0[qs].y * hv);
-        X1[qs] = make_float2(X1[qs].x * hv, X1[qs].y * hv);
-    }
-    __syncthreads();
-    h = 1;
-    if (rem == 1) {
-        fft_dit2(X0, X1, n, tid);
-        h = 2;
-    } else if (rem == 2) {
-        fft_dit4<true>(X0, X1, n, 1, T, tid);
-        h = 4;
-    }
-    for (; h < n; h <<= 3) fft_dit8<true>(X0, X1, n, h, T, tid);
+    // operations as the separate stages with one LDS round trip).
+    // The innermost 3, 1 or 2 stages of the forward transform, the product
+    // with the real, even filter (H[n-k] = H[k]) and the same stages of the
+    // inverse run as one pass on groups of G = 8, 2 or 4 consecutive
+    // elements; the other stages run as fused radix-8 passes either side.
+    constexpr int rem = logn % 3;
+    constexpr int G = rem == 0 ? 8 : (1 << rem);
+    dif8_passes<n, n / 8, G>(X0, X1, T, tid);
+    fft_middle<logn, G>(X0, X1, T, H, tid);
+    dit8_passes<n, G>(X0, X1, T, tid);
     const float inv_n = 1.0f / (float)n;
     g_float4* dst = (g_float4*)p.filt + ((size_t)zb * p.na + a) * p.sq;
     for (int i = tid; i < p.sq; i += 256) {
@@ -325,13 +333,16 @@ __device__ __forceinline__ void fft_filter_item(const FiltArgs& p, int a, int zb
     __syncthreads();  // LDS is reused by the next item
 }
 
+template <int LOGN>
 __global__ __launch_bounds__(256) void row_filter_fft_kernel(FiltArgs p) {
     extern __shared__ float2 sm[];
-    fft_filter_item(p, blockIdx.x, blockIdx.y, sm);
+    fft_filter_item<LOGN>(p, blockIdx.x, blockIdx.y, sm);
 }
 
-// Persistent form: items (job, angle, z-block) of many volumes, taken from an
-// atomic counter until the list is exhausted (every block then returns).
+// Persistent form: items (job, angle, z-block) of many volumes with one FFT
+// length 2^LOGN, taken from an atomic counter until the list is exhausted
+// (every block then returns).
+template <int LOGN>
 __global__ __launch_bounds__(256) void row_filter_persistent_kernel(const FiltJob* __restrict__ jobs,
                                                                     const int* __restrict__ hdr,
                                                                     int* __restrict__ counter) {
@@ -352,7 +363,7 @@ __global__ __launch_bounds__(256) void row_filter_persistent_kernel(const FiltJo
         const FiltJob& jb = jobs[j];
         const int local = item - jb.item_begin;
         const int zb = local / jb.a.na, a = local - zb * jb.a.na;
-        fft_filter_item(jb.a, a, zb, sm);
+        fft_filter_item<LOGN>(jb.a, a, zb, sm);
     }
 }
 
@@ -363,28 +374,65 @@ static hipError_t set_lds_attr(const void* fn) {
                                (int)fft_lds_bytes(kMaxFFT));
 }
 
-int filter_persistent_grid(int max_nfft, int device) {
+template <int LOGN>
+static int persistent_grid_n(int device) {
     int per_cu = 0, cus = 0;
-    if (set_lds_attr((const void*)row_filter_persistent_kernel) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)row_filter_persistent_kernel,
-                                                     256, fft_lds_bytes(max_nfft)) != hipSuccess)
+    if (set_lds_attr((const void*)row_filter_persistent_kernel<LOGN>) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, (const void*)row_filter_persistent_kernel<LOGN>, 256,
+            fft_lds_bytes(1 << LOGN)) != hipSuccess)
         return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return 0;
     return (per_cu < 1 ? 1 : per_cu) * cus;
 }
 
-hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
-                                        int max_nfft, int grid, hipStream_t s) {
+template <int LOGN>
+static hipError_t launch_persistent_n(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
+                                      int grid, hipStream_t s) {
     static bool attr = false;
     if (!attr) {
-        const hipError_t e = set_lds_attr((const void*)row_filter_persistent_kernel);
+        const hipError_t e = set_lds_attr((const void*)row_filter_persistent_kernel<LOGN>);
         if (e != hipSuccess) return e;
         attr = true;
     }
-    hipLaunchKernelGGL(row_filter_persistent_kernel, dim3(grid), dim3(256), fft_lds_bytes(max_nfft),
-                       s, d_jobs, d_hdr, d_counter);
+    hipLaunchKernelGGL(row_filter_persistent_kernel<LOGN>, dim3(grid), dim3(256),
+                       fft_lds_bytes(1 << LOGN), s, d_jobs, d_hdr, d_counter);
     return hipGetLastError();
+}
+
+static int log2_of(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+// nfft: the FFT length of every job in the launch's list (64 .. kMaxFFT).
+int filter_persistent_grid(int nfft, int device) {
+    switch (log2_of(nfft)) {
+        case 6: return persistent_grid_n<6>(device);
+        case 7: return persistent_grid_n<7>(device);
+        case 8: return persistent_grid_n<8>(device);
+        case 9: return persistent_grid_n<9>(device);
+        case 10: return persistent_grid_n<10>(device);
+        case 11: return persistent_grid_n<11>(device);
+        case 12: return persistent_grid_n<12>(device);
+        default: return 0;
+    }
+}
+
+hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
+                                        int nfft, int grid, hipStream_t s) {
+    switch (log2_of(nfft)) {
+        case 6: return launch_persistent_n<6>(d_jobs, d_hdr, d_counter, grid, s);
+        case 7: return launch_persistent_n<7>(d_jobs, d_hdr, d_counter, grid, s);
+        case 8: return launch_persistent_n<8>(d_jobs, d_hdr, d_counter, grid, s);
+        case 9: return launch_persistent_n<9>(d_jobs, d_hdr, d_counter, grid, s);
+        case 10: return launch_persistent_n<10>(d_jobs, d_hdr, d_counter, grid, s);
+        case 11: return launch_persistent_n<11>(d_jobs, d_hdr, d_counter, grid, s);
+        case 12: return launch_persistent_n<12>(d_jobs, d_hdr, d_counter, grid, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_twiddles(int n, float2* d_tw, hipStream_t s) {
@@ -411,20 +459,34 @@ FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, i
     return p;
 }
 
-hipError_t launch_row_filter_fft(const float* d_sino, int nz, int na, int bins, int sq, int pad,
-                                 int nfft, const float* d_H, const float2* d_tw, float4* d_filt,
-                                 hipStream_t s) {
-    if (nfft < 4 || nfft > kMaxFFT || (nfft & (nfft - 1))) return hipErrorInvalidValue;
+template <int LOGN>
+static hipError_t launch_fft_n(const FiltArgs& p, int nzb, hipStream_t s) {
     static bool attr_set = false;  // > 64 KB of dynamic LDS must be opted into
     if (!attr_set) {
-        const hipError_t e = set_lds_attr((const void*)row_filter_fft_kernel);
+        const hipError_t e = set_lds_attr((const void*)row_filter_fft_kernel<LOGN>);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
+    hipLaunchKernelGGL(row_filter_fft_kernel<LOGN>, dim3(p.na, nzb), dim3(256),
+                       fft_lds_bytes(1 << LOGN), s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_filter_fft(const float* d_sino, int nz, int na, int bins, int sq, int pad,
+                                 int nfft, const float* d_H, const float2* d_tw, float4* d_filt,
+                                 hipStream_t s) {
+    if (nfft < 64 || nfft > kMaxFFT || (nfft & (nfft - 1))) return hipErrorInvalidValue;
     const int nzb = (nz + kZB - 1) / kZB;
     const FiltArgs p = make_filt_args(d_sino, nz, na, bins, sq, pad, nfft, d_H, d_tw, d_filt);
-    hipLaunchKernelGGL(row_filter_fft_kernel, dim3(na, nzb), dim3(256), fft_lds_bytes(nfft), s, p);
-    return hipGetLastError();
+    switch (p.logn) {
+        case 6: return launch_fft_n<6>(p, nzb, s);
+        case 7: return launch_fft_n<7>(p, nzb, s);
+        case 8: return launch_fft_n<8>(p, nzb, s);
+        case 9: return launch_fft_n<9>(p, nzb, s);
+        case 10: return launch_fft_n<10>(p, nzb, s);
+        case 11: return launch_fft_n<11>(p, nzb, s);
+        default: return launch_fft_n<12>(p, nzb, s);
+    }
 }
 
 }  // namespace cthip
