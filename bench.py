@@ -39,8 +39,8 @@ FLOP_PER_SAMPLE = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows_kernel": 1.0643e+10,
-    "bp_kernel": 6.524e+09,
+    "radon_rows_kernel": 1.1133e+10,
+    "bp_kernel": 6.458e+09,
 }
 
 
