@@ -85,9 +85,8 @@ void batch_release(ct_hip_ctx* c) {
 }
 
 static void load_tune_cache(ct_hip_ctx* c) {
-    static bool loaded = false;
-    if (loaded) return;
-    loaded = true;
+    if (c->tune_cache_loaded) return;
+    c->tune_cache_loaded = true;
     const char* path = std::getenv("CT_HIP_TUNE_CACHE");
     if (!path) return;
     std::ifstream f(path);

@@ -101,6 +101,7 @@ struct ct_hip_ctx {
     Buf rmax;
     // batched form (batch.cpp)
     std::map<std::pair<int, int>, int> tuned;  // (N, na) -> backprojection config
+    bool tune_cache_loaded = false;            // CT_HIP_TUNE_CACHE read into `tuned`
     cthost::BatchGraph* batch = nullptr;
 };
 

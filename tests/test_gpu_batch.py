@@ -112,3 +112,18 @@ def test_graph_replay_with_changing_class_composition(O):
             s, ds, do = pool[(N, nz, na)]
             ref = _alone(be, s, O.serial_angles(na), ct.FilterType.Ramp)
             assert np.array_equal(do.download((nz, N, N)), ref), (comp, N)
+
+
+def test_tune_cache_file_is_read_by_every_context(tmp_path, monkeypatch):
+    """CT_HIP_TUNE_CACHE entries apply to each new context, not only the
+    first one of the process."""
+    import ctypes
+    cache = tmp_path / "tune.txt"
+    cache.write_text("48 30 6\n")
+    monkeypatch.setenv("CT_HIP_TUNE_CACHE", str(cache))
+    for _ in range(2):
+        ctx = ct.Context(0)
+        best = ctypes.c_int(-1)
+        ctx._call("ct_hip_autotune", ctx.handle, 48, 30, ctypes.byref(best), None, 0)
+        assert best.value == 6
+        ctx.close()
