@@ -24,6 +24,7 @@
 
 #include <cctype>
 #include <cfloat>
+#include <charconv>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -91,6 +92,17 @@ std::string read_npy_header(std::ifstream& in, const std::string& path) {
     return h;
 }
 
+// Number of elements of a shape; CT_ERR_IO when it overflows size_t.
+size_t checked_count(const size_t* dims, int k, const std::string& path) {
+    size_t n = 1;
+    for (int i = 0; i < k; ++i) {
+        if (dims[i] != 0 && n > std::numeric_limits<size_t>::max() / 4 / dims[i])
+            throw IoError("NPY shape too large: " + path);
+        n *= dims[i];
+    }
+    return n;
+}
+
 // The first k numbers between the first '(' and the following ')'.
 bool parse_shape(const std::string& h, int k, size_t* dims) {
     const size_t l = h.find('(');
@@ -112,16 +124,15 @@ void read_floats(std::ifstream& in, float* dst, size_t n, const std::string& pat
 
 // ----------------------------------------------------------------- JSON
 
-// Shortest decimal form that reads back as the same double; null for
+// Shortest decimal form that reads back as the same double (std::to_chars:
+// independent of the C locale, which a Qt application sets from the
+// environment, where a decimal comma would make invalid JSON); null for
 // non-finite values, which JSON cannot hold.
 std::string json_number(double v) {
     if (!std::isfinite(v)) return "null";
-    char buf[40];
-    for (int p = 1; p <= 17; ++p) {
-        std::snprintf(buf, sizeof buf, "%.*g", p, v);
-        if (std::strtod(buf, nullptr) == v) break;
-    }
-    return buf;
+    char buf[64];
+    const std::to_chars_result r = std::to_chars(buf, buf + sizeof buf, v);
+    return std::string(buf, r.ptr);
 }
 
 struct JVal {
@@ -190,7 +201,20 @@ private:
             while (i_ < s_.size() && std::isdigit((unsigned char)s_[i_])) ++i_;
         }
         v.kind = JVal::Num;
-        v.num = std::strtod(s_.substr(b, i_ - b).c_str(), nullptr);
+        // from_chars takes no leading '+' and reads '-' itself; the grammar
+        // above has checked the form. Magnitudes past double range read as
+        // +-inf, as strtod would give.
+        const std::from_chars_result r = std::from_chars(s_.data() + b, s_.data() + i_, v.num);
+        if (r.ec == std::errc::result_out_of_range) {
+            const bool neg = s_[b] == '-';
+            size_t e = s_.find_first_of("eE", b);
+            const bool tiny = e != std::string::npos && e + 1 < i_ && s_[e + 1] == '-';
+            v.num = tiny ? (neg ? -0.0 : 0.0)
+                         : (neg ? -std::numeric_limits<double>::infinity()
+                                : std::numeric_limits<double>::infinity());
+        } else if (r.ec != std::errc()) {
+            return false;
+        }
         return true;
     }
     static void utf8(std::string& o, unsigned cp) {
@@ -365,6 +389,7 @@ int ct_hip_load_volume_npy(const char* path, int input_is_xyz_layout, float* vol
         size_t s[3];
         if (!parse_shape(h, 3, s) || s[0] == 0 || s[1] == 0 || s[2] == 0)
             throw IoError(std::string("NPY shape is not three non-zero sizes: ") + path);
+        checked_count(s, 3, path);
         // the Volume the reference builds: (z,y,x) files keep their order,
         // width = last axis; (x,y,z) files are transposed, width = first axis
         if (input_is_xyz_layout) {
@@ -377,7 +402,7 @@ int ct_hip_load_volume_npy(const char* path, int input_is_xyz_layout, float* vol
             dims3[2] = s[0];
         }
         if (!vol) return;
-        const size_t n = s[0] * s[1] * s[2];
+        const size_t n = checked_count(s, 3, path);
         if (capacity < n) throw StatusError(CT_ERR_INVALID, "loadVolumeNPY: buffer too small");
         if (!input_is_xyz_layout) {
             read_floats(in, vol, n, path);
@@ -406,6 +431,7 @@ int ct_hip_load_point_cloud_npy(const char* path, float* xyzh, size_t capacity, 
         size_t s[2];
         if (!parse_shape(h, 2, s) || s[0] == 0 || s[1] != 4)
             throw IoError(std::string("NPY shape is not (n, 4): ") + path);
+        checked_count(s, 2, path);
         *n = s[0];
         if (!xyzh) return;
         if (capacity < s[0]) throw StatusError(CT_ERR_INVALID, "loadPointCloudNPY: buffer too small");

@@ -340,3 +340,22 @@ def test_export_and_load_sinograms(O, backend, tmp_path):
     b = backend.reconstructVolumeFromSinograms(s, np.float32(step) * np.arange(na, dtype=np.float32),
                                                params)
     np.testing.assert_array_equal(a, b)
+
+
+def test_sidecar_number_forms(tmp_path):
+    """JSON numbers in exponent form, an underflowing one and a negative zero;
+    a malformed number makes the sidecar unparsable (every default)."""
+    p, s = _write_stack(tmp_path, shape=(1, 4, 8))
+    with open(_sidecar(p), "w") as f:
+        f.write('{"angles_deg": [2.5E+1, 1e-400, -0.0, 4.5e-1],'
+                ' "detector_spacing_mm": 125e-3, "original_min_hu": -1.5E3, "original_max_hu": 2e3}')
+    L = ct.loadSinograms(p)
+    assert L.meta_found and L.meta_angles and L.meta_minmax
+    np.testing.assert_array_equal(L.angles_deg, np.array([25.0, 0.0, -0.0, 0.45], np.float32))
+    assert L.detector_spacing_mm == 0.125 and L.min_hu == -1500.0 and L.max_hu == 2000.0
+    for bad in ('{"detector_spacing_mm": 01}', '{"detector_spacing_mm": 1.}',
+                '{"detector_spacing_mm": +1}', '{"detector_spacing_mm": .5}'):
+        with open(_sidecar(p), "w") as f:
+            f.write(bad)
+        L = ct.loadSinograms(p)
+        assert not L.meta_found and L.detector_spacing_mm == 1.0, bad
