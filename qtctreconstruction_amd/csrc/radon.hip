@@ -6,19 +6,22 @@
 // v = (raw - min) * (1/span), and deposits v*(1-frac) / v*frac into bins
 // i0 = floor(s), i0+1 (full v when i0 == bins-1, nothing when i0 < 0).
 //
-// Here every lane owns one detector bin of one angle (a ray) and gathers: it
-// visits, in the same raster order, exactly the pixels whose floor(s) is its
-// bin or the bin below, recomputing s with Serial's float expression. Each
-// bin therefore receives the same float additions in the same order as in the
-// scatter, so the sinogram is bit-identical to Serial's, with no atomics.
-// The candidate pixels come from a conservative interval in s (+-1/32 of a
-// bin); the exact floor(s) test decides.
+// Here every lane owns one unit interval [i, i+1) of s at one angle and
+// gathers: it visits, row by row, exactly the pixels whose floor(s) is i,
+// recomputing s with Serial's float expression, so every pixel's bin and
+// fraction are Serial's. The lane sums the pixels' first deposits (for bin
+// i) and their second ones (for bin i+1) apart, with fused multiply-adds,
+// and bin i is the first sum of lane i plus the second of lane i-1: no
+// atomics. A bin with a single deposit is Serial's rounded product bit for
+// bit; sums of several differ from Serial's raster-order sums by rounding
+// only. The candidate pixels come from a conservative interval in s
+// (+-1/32 of a bin); the exact floor(s) test decides.
 //
 // Eight slices share one lane: the geometry is slice-independent, so the index
-// work is done once and the eight values arrive as two float4 loads. Rays of
-// angles nearer 0/180 deg walk down columns of the row-major copy; rays of
-// angles nearer 90 deg walk along rows, and read the transposed copy, so that
-// adjacent lanes (adjacent bins) read adjacent addresses in both cases.
+// work is done once and the eight values arrive as two float4 loads. Angles
+// with |sin| <= |cos| walk the rows of the row-major copy; the others walk
+// the rows of the transposed copy, so that adjacent lanes read adjacent
+// pixels in both cases.
 #include "ct_internal.h"
 
 #include <algorithm>
@@ -201,21 +204,23 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
 
 // s-interval margin: the float s of SerialBackend.cpp:110 is within ~3e-4 of
 // the exact value for slices up to 2048 px, so a candidate interval built
-// from [i-1-kDelta, i+1+kDelta] in exact arithmetic holds every pixel with a
-// nonzero factor for bin i.
+// from [i-kDelta, i+1+kDelta] in exact arithmetic holds every pixel with
+// floor(s) = i.
 constexpr float kDelta = 1.0f / 32.0f;
 
-// Lane geometry of one bin over an image walked by rows, for an angle with
-// |sn| <= |c|. Row y holds the lane's candidates in [xa(y), xa(y) + span],
-// xa(y) = xa0 + y*dxa: the bin's s-interval over the row, stepped from row
-// to row (|dxa| <= 1 and |1/c| <= sqrt2, so the stepping error, ~1e-3 px, is
-// far below the kDelta/|c| and 0.01 px margins), at most span + 0.02 <=
-// 2.0625*sqrt2 + 0.02 < 3 pixels.
+// Lane geometry of one interval over an image walked by rows, for an angle
+// with |sn| <= |c|. Lane i owns the pixels with floor(s) = i. Row y holds
+// them in [xa(y), xa(y) + span], xa(y) = xa0 + y*dxa: the interval's
+// x-range over the row, stepped from row to row (|dxa| <= 1 and
+// |1/c| <= sqrt2, so the stepping error, ~1e-3 px, is far below the
+// kDelta/|c| and 0.01 px margins), at most span + 0.02 <= 1.0625*sqrt2 +
+// 0.02 < 2 pixels: two candidates per row.
 struct RayGeom {
     float c, sn, cx, cy, dc;
     float xa0, dxa;  // interval start at row 0, less 0.01 px, and its step
     int w;
-    // factor of this lane's bin: clamp(min(s*ka + la, s*kb + lb), 0, 1)
+    float fi;        // the interval's bin i, as a float
+    // weights of a pixel at u = s - i (the interval_weights rules)
     float ka, la, kb, lb;
 };
 
@@ -226,64 +231,70 @@ __device__ __forceinline__ float row_first(const RayGeom& g, float yf) {
     return ceilf(__builtin_amdgcn_fmed3f(fmaf(yf, g.dxa, g.xa0), 0.0f, (float)g.w));
 }
 
-// Factor of bin i for a pixel at s. Serial (SerialBackend.cpp:110-120) adds
-// v*(1-frac) to bin i0 = floor(s) and v*frac to bin i0+1, frac = s - i0; the
-// last bin takes v itself when i0 is the last bin; nothing when i0 < 0. With
-//   a = s - (i-1),  b = (i+1) - s,  fac = clamp(min(a, b), 0, 1):
-// - s in [i-1, i): Serial's weight is frac = fl(s - (i-1)), the same float
-//   operation as a, and a <= 1 <= b;
-// - s in [i, i+1): Serial's is fl(1 - fl(s - i)); s - i is exact (Sterbenz
-//   for i >= 1, trivially for i = 0), so both round the same real number
-//   (i+1) - s, and b <= 1 <= a;
-// - elsewhere a <= 0 or b <= 0 (rounding is monotone), so fac = 0.
-// Bin 0 takes nothing for s < 0: its a = s*2^64 + 1 is >= 1 for s >= 0 and
-// negative for s < 0 (a nonzero s is a multiple of 2^-49: |cos|, |sin| are 0
-// or above 2^-25 and pixel offsets are multiples of 1/2). The last bin takes
-// v for s in [i, i+1): its b = (i+1-s)*2^64 is >= 1 there (s is a float
-// below i+1) and <= 0 from i+1 on, so the clamp gives exactly 1. Each of a
-// and b is one fma: fl(s*1 + c) = fl(s + c). The products v*fac are rounded
-// before they are added, as in Serial; a zero factor adds +0 or -0, which
-// leaves a partial sum >= +0 unchanged.
-__device__ __forceinline__ void bin_factors(RayGeom& g, int i, int bins) {
+// Serial (SerialBackend.cpp:110-120) adds, for every pixel with i0 = floor(s)
+// and frac = s - i0, v*(1-frac) to bin i0 and v*frac to bin i0+1 when
+// 0 <= i0 < bins-1, v to bin i0 when i0 = bins-1, nothing otherwise. Lane i
+// takes the pixels with i0 = i: it sums their first deposits (A, for bin i)
+// and their second ones (B, for bin i+1); bin i is then A_i + B_(i-1).
+// With u = s - i (exact: Sterbenz where s is near i; where it is not, u is
+// far from [0, 1) anyway), u in [0, 1) is exactly i0 = i, u is Serial's frac
+// and fl(1 - u) its 1 - frac. A nonzero s is a multiple of 2^-49 (|cos|,
+// |sin| are 0 or above 2^-25, pixel offsets multiples of 1/2), so is u, and:
+//   A = clamp(min(fma(u, ka, la), fma(u, 2^64, 1)), 0, 1)
+//   B = clamp(min(u,              fma(u, kb, lb)),  0, 1)
+// - interior lanes (ka, la) = (-1, 1): fl(1 - u) for u in [0, 1), <= 0 for
+//   u >= 1; (kb, lb) = (-2^64, 2^64): 2^64 (1 - u) >= 2^15 for u < 1 and
+//   <= 0 for u >= 1; the second operand of A, u*2^64 + 1, is >= 1 for
+//   u >= 0 and <= 1 - 2^15 for u < 0, where B's u is negative too;
+// - the last lane, i = bins-1: (ka, la) = (-2^64, 2^64) gives A = 1 on
+//   [0, 1) and 0 elsewhere; (kb, lb) = (0, 0) gives B = 0;
+// - lanes past the last bin: (ka, la) = (kb, lb) = (0, -1), A = B = 0.
+// Each weight is two or three VALU (fma, fma, min with clamp).
+__device__ __forceinline__ void interval_weights(RayGeom& g, int i, int bins) {
     constexpr float kBig = 18446744073709551616.0f;  // 2^64
-    g.ka = 1.0f;
-    g.la = -(float)(i - 1);
-    g.kb = -1.0f;
-    g.lb = (float)(i + 1);
-    if (i == 0) {  // SerialBackend.cpp:113 (i0 >= 0)
-        g.ka = kBig;
-        g.la = 1.0f;
-    }
+    g.fi = (float)i;
+    g.ka = -1.0f;
+    g.la = 1.0f;
+    g.kb = -kBig;
+    g.lb = kBig;
     if (i == bins - 1) {  // SerialBackend.cpp:117-118
-        g.kb = -kBig;
-        g.lb = (float)(i + 1) * kBig;
+        g.ka = -kBig;
+        g.la = kBig;
+        g.kb = 0.0f;
+        g.lb = 0.0f;
     }
 }
 
 // The candidates x0 + k, k < U, of one row; xm0 = x0 - cx exactly (pixel
-// offsets are multiples of 1/2 far below 2^24).
+// offsets are multiples of 1/2 far below 2^24). The deposits are fused
+// multiply-adds: every partial sum is within a rounding of Serial's terms,
+// and a bin with one deposit gets Serial's rounded product exactly.
 template <int U>
 __device__ __forceinline__ void chunk_add(const RayGeom& g, float xm0, float sy,
-                                          const float4 (&v)[U][2], float (&acc)[kRZ]) {
+                                          const float4 (&v)[U][2], float (&accA)[kRZ],
+                                          float (&accB)[kRZ]) {
+    constexpr float kBig = 18446744073709551616.0f;  // 2^64
 #pragma unroll
     for (int k = 0; k < U; ++k) {
         // SerialBackend.cpp:110, same float operations and order (xm0 is
         // never -0, so xm0 + 0 is xm0)
         const float s = (k == 0 ? xm0 : xm0 + (float)k) * g.c + sy + g.dc;
-        const float a = fmaf(s, g.ka, g.la), b = fmaf(s, g.kb, g.lb);
-        const float fac = __builtin_amdgcn_fmed3f(fminf(a, b), 0.0f, 1.0f);
-        acc[0] += v[k][0].x * fac;
-        acc[1] += v[k][0].y * fac;
-        acc[2] += v[k][0].z * fac;
-        acc[3] += v[k][0].w * fac;
-        acc[4] += v[k][1].x * fac;
-        acc[5] += v[k][1].y * fac;
-        acc[6] += v[k][1].z * fac;
-        acc[7] += v[k][1].w * fac;
+        const float u = s - g.fi;
+        const float wa = __builtin_amdgcn_fmed3f(fminf(fmaf(u, g.ka, g.la), fmaf(u, kBig, 1.0f)),
+                                                 0.0f, 1.0f);
+        const float wb = __builtin_amdgcn_fmed3f(fminf(u, fmaf(u, g.kb, g.lb)), 0.0f, 1.0f);
+        const float p[kRZ] = {v[k][0].x, v[k][0].y, v[k][0].z, v[k][0].w,
+                              v[k][1].x, v[k][1].y, v[k][1].z, v[k][1].w};
+#pragma unroll
+        for (int z = 0; z < kRZ; ++z) {
+            accA[z] = fmaf(p[z], wa, accA[z]);
+            accB[z] = fmaf(p[z], wb, accB[z]);
+        }
     }
 }
 
-// Lane geometry of bin i at angle (c, sn), |sn| <= |c|, over a w x h image.
+// Lane geometry of interval i at angle (c, sn), |sn| <= |c|, over a w x h
+// image.
 __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float c, float sn) {
     RayGeom g;
     g.c = c;
@@ -292,9 +303,9 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
     g.cy = (float)(h - 1) * 0.5f;
     g.dc = (float)(bins / 2);
     g.w = w;
-    bin_factors(g, i, bins);
+    interval_weights(g, i, bins);
     // x - cx = ((s - dc) - (y - cy) sn) / c over s - dc in [slo, shi]
-    const float slo = (float)(i - 1) - kDelta - g.dc;
+    const float slo = (float)i - kDelta - g.dc;
     const float shi = (float)(i + 1) + kDelta - g.dc;
     const float inv_c = 1.0f / c;
     const float e0 = (slo + g.cy * sn) * inv_c, e1 = (shi + g.cy * sn) * inv_c;
@@ -303,16 +314,25 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
     return g;
 }
 
+// Intervals per bin block: a block of NT lanes takes the intervals
+// [j(NT-1), j(NT-1) + NT) and writes the bins of its lanes 1.. (lane 0 too
+// in the first block), since bin i needs lane i-1's B. Neighbouring blocks
+// share one interval.
+__host__ __device__ inline int radon_bin_blocks(int bins, int NT) {
+    const int b = (bins - 1 + NT - 2) / (NT - 1);
+    return b < 1 ? 1 : b;
+}
+
 // Row walk over a w x h image copy [zb][h][w + kPadX] for angles with
-// |sin| <= |cos|: every lane of a block (256 bins of one angle and z-block)
-// walks all rows top to bottom with one chunk of U = 3 candidates per row.
-// The block stages RR rows at a time into LDS, each as a window of kRowW
-// pixels starting at the extreme lane's first candidate. A lane's first
+// |sin| <= |cos|: every lane of a block (NT intervals of one angle and
+// z-block) walks all rows top to bottom with one chunk of U = 2 candidates
+// per row. The block stages RR rows at a time into LDS, each as a window of
+// kRowW pixels starting at the extreme lane's first candidate. A lane's first
 // candidate x0(y) is a composition of monotone float operations of its bin,
 // so the window start is that of the extreme bin (the lowest one for
 // cos > 0, the highest for cos < 0), and every lane's chunk lies within
 // kRowW pixels of it: lanes 255 bins apart are at most 255/|cos| <= 360.7
-// pixels apart, plus a chunk of 3. Rows outside a lane's candidate rows add
+// pixels apart, plus a chunk of 2. Rows outside a lane's candidate rows add
 // +0.
 //
 // The other angles walk the transposed image: the column copy
@@ -320,13 +340,12 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
 // it with (w, h) and (cos, sin) exchanged. Serial's s = (x-cx)*cos +
 // (y-cy)*sin + dc is then formed as (y-cy)*sin + (x-cx)*cos + dc, the same
 // float value (the first addition commutes exactly), so every pixel's bin and
-// fraction are Serial's; each bin sums its deposits column by column instead
-// of row by row.
+// fraction are Serial's.
 constexpr int kRadonRows = 2;  // rows staged per step
 
 // Workgroups follow the XCD-grouped order (xcd_map): units are z-blocks,
 // a z-block's slots its (bin block, angle) pairs, n_ang angles. A block has
-// NT lanes (NT bins). FULL: the image row fits the window (w + kPadX <=
+// NT lanes (NT intervals). FULL: the image row fits the window (w + kPadX <=
 // kRowW), every block stages whole rows and the window start is 0; NT = 512
 // then, 8 waves sharing 2 staged rows (32 waves per CU at 516-pixel rows,
 // against 24 for the windowed 256-lane form, whose LDS per wave is larger).
@@ -342,29 +361,30 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
     static_assert(FULL || (NT - 1) * 1.41421357 + U <= kRowW,
                   "window narrower than the lanes' span");
     static_assert(RR == 2 && NT % 128 == 0, "two staged rows, one per half of the block");
+    static_assert(RR * kRowW >= 2 * NT, "the B exchange reuses the staging buffers");
     __shared__ float4 t0[RR][kRowW];  // slices 0-3 of the staged pixels
     __shared__ float4 t1[RR][kRowW];  // slices 4-7
     __shared__ int xw_s[2][RR];       // window starts: staged rows, next rows
-    const int bb = (bins + NT - 1) / NT;
+    const int bb = radon_bin_blocks(bins, NT);
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int slots = (bb * n_ang + gsz - 1) / gsz;
     int zb, slot;
     if (!xcd_map(blockIdx.x, gsz, slots, bb * n_ang, nzb, zb, slot)) return;  // whole block
     const int a = alist[slot / bb];
-    const int b0 = (slot % bb) * NT;
+    const int b0 = (slot % bb) * (NT - 1);
     const int i = b0 + threadIdx.x;
     const bool live = i < bins;
     const float c = cos_t[a], sn = sin_t[a];
     RayGeom g = ray_geom(live ? i : bins - 1, bins, w, h, c, sn);
-    if (!live) {  // walks along, adds nothing: factor clamp(-1) = 0
+    if (!live) {  // walks along, adds nothing
         g.ka = g.kb = 0.0f;
         g.la = g.lb = -1.0f;
     }
     const int wp = w + kPadX;
     const char* __restrict__ slab =
         reinterpret_cast<const char*>(rowL) + (size_t)zb * radon_zbytes(w, h);
-    // the extreme bin's geometry, held by the first RR lanes only: lane r
-    // computes the window start of row ys + r of the next step
+    // the extreme interval's geometry, held by the first RR lanes only: lane
+    // r computes the window start of row ys + r of the next step
     const int ie = c > 0.0f ? b0 : min(b0 + NT - 1, bins - 1);
     RayGeom ge;
     if (threadIdx.x < RR) {
@@ -385,9 +405,9 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
     const int sj0 = threadIdx.x % HB;
     const int nstage = FULL ? wp : kRowW;
 
-    float acc[kRZ];
+    float accA[kRZ], accB[kRZ];
 #pragma unroll
-    for (int k = 0; k < kRZ; ++k) acc[k] = 0.0f;
+    for (int k = 0; k < kRZ; ++k) accA[k] = accB[k] = 0.0f;
     int buf = 0;
     for (int ys = 0; ys < h; ys += RR, buf ^= 1) {
         __syncthreads();  // the previous rows are consumed; xw_s[buf] is set
@@ -430,14 +450,36 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
                 v[k][0] = t0[r][d + k];
                 v[k][1] = t1[r][d + k];
             }
-            chunk_add<U>(g, x0f - g.cx, sy, v, acc);
+            chunk_add<U>(g, x0f - g.cx, sy, v, accA, accB);
         }
     }
-    if (live) {
+    // bin i = A_i + B_(i-1): every lane leaves its B in LDS (the staging
+    // buffers, consumed), lane t > 0 adds lane t-1's; lane 0 writes only in
+    // the first bin block, where bin 0 has no B before it
+    __syncthreads();
+    float4* bx = &t0[0][0];
+    bx[2 * threadIdx.x] = make_float4(accB[0], accB[1], accB[2], accB[3]);
+    bx[2 * threadIdx.x + 1] = make_float4(accB[4], accB[5], accB[6], accB[7]);
+    __syncthreads();
+    if (live && (threadIdx.x > 0 || b0 == 0)) {
+        float pb[kRZ];
+#pragma unroll
+        for (int k = 0; k < kRZ; ++k) pb[k] = 0.0f;
+        if (threadIdx.x > 0) {
+            const float4 p0 = bx[2 * threadIdx.x - 2], p1 = bx[2 * threadIdx.x - 1];
+            pb[0] = p0.x;
+            pb[1] = p0.y;
+            pb[2] = p0.z;
+            pb[3] = p0.w;
+            pb[4] = p1.x;
+            pb[5] = p1.y;
+            pb[6] = p1.z;
+            pb[7] = p1.w;
+        }
 #pragma unroll
         for (int k = 0; k < kRZ; ++k) {
             const int z = zb * kRZ + k;
-            if (z < nz) sino[((size_t)z * na + a) * bins + i] = acc[k];
+            if (z < nz) sino[((size_t)z * na + a) * bins + i] = accA[k] + pb[k];
         }
     }
 }
@@ -481,15 +523,15 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
     const int nzb = (nz + kRZ - 1) / kRZ;
     const bool full = w + kPadX <= 520;  // whole rows fit the 512-lane form's window
     const int NT = full ? 512 : 256;
-    const int bb = (bins + NT - 1) / NT;
+    const int bb = radon_bin_blocks(bins, NT);
     const int gsz = xcd_group(bb * n, nzb);
     const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
     const dim3 grid((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
     if (full)
-        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 520, 512, true>), grid, dim3(512), 0,
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 520, 512, true>), grid, dim3(512), 0,
                            s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n, gsz);
     else
-        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 3, 368, 256, false>), grid, dim3(256),
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 368, 256, false>), grid, dim3(256),
                            0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n, gsz);
     return hipGetLastError();
 }

@@ -1,12 +1,14 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
 Bars: phantom voxels bit-exact; every pixel's detector bin and fraction
-bit-exact (Serial's float expression for s); sinograms within relative L2
-<= 1e-5 of Serial and bit-exact for the angles with |sin| <= |cos|, whose
-bins sum in Serial's raster order (steeper angles sum column by column);
-filtered reconstructions within relative L2 <= 1e-5 of the Serial
-restatement (FP32 FFT vs Serial's FP32 FFT, FMA interpolation); every
-reconstruction also reports its error vs the FP64 oracle.
+bit-exact (Serial's float expression for s; a bin with one deposit is
+Serial's rounded product bit for bit); sinograms within relative L2 <= 1e-5
+of Serial (each lane sums its interval's two deposit kinds apart with fused
+multiply-adds, so sums of several deposits differ from Serial's raster-order
+sums by rounding only, ~5e-7); filtered reconstructions within relative L2
+<= 1e-5 of the Serial restatement (FP32 FFT vs Serial's FP32 FFT, FMA
+interpolation); every reconstruction also reports its error vs the FP64
+oracle.
 """
 import threading
 
@@ -25,23 +27,15 @@ def _bits_equal(a, b):
                           np.asarray(b, np.float32).view(np.uint32))
 
 
-def _raster_angles(na):
-    """Serial angles whose bins sum in raster order (|sin| <= |cos| in the
-    float32 tables), leaving out those within 1e-6 of 45 degrees, where a
-    last-bit difference between trig libraries could move the angle."""
-    th = (np.arange(na, dtype=np.float32) * np.float32(180.0 / na)) * np.float32(np.pi / 180.0)
-    c = np.abs(np.cos(th.astype(np.float64)).astype(np.float32))
-    s = np.abs(np.sin(th.astype(np.float64)).astype(np.float32))
-    return np.nonzero((s <= c) & (np.abs(s - c) > 1e-6))[0]
+SINO_TOL = 2e-6  # observed ~5e-7: rounding of the fused, split sums only
 
 
 def _sino_parity(O, got, ref):
-    """Raster-order angles bit-identical, every angle within TOL of Serial."""
+    """Every angle within SINO_TOL (below the 1e-5 bar) of Serial."""
     got = np.asarray(got, np.float32)
-    rows = _raster_angles(got.shape[-2])
-    assert _bits_equal(got[..., rows, :], np.asarray(ref, np.float32)[..., rows, :])
+    assert np.isfinite(got).all()
     err = O.rel_l2(got, ref)
-    assert err <= TOL, err
+    assert err <= SINO_TOL, err
     return err
 
 
