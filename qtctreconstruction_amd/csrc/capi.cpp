@@ -174,6 +174,9 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
     float4* row = ensure<float4>(c, c->row, rzb * radon_zbytes(w, h));
     float4* col = ensure<float4>(c, c->col, rzb * radon_zbytes(h, w));
+    const size_t nside = std::max(radon_side_floats(w, bins, nz, na),
+                                  radon_side_floats(h, bins, nz, na));
+    float* side = nside ? ensure<float>(c, c->side, sizeof(float) * nside) : nullptr;
     hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
@@ -184,11 +187,11 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     hip_check(hipEventRecord(c->evr[0], c->stream), "event");
     const int n_row = c->cur.n_row;
     hip_check(launch_radon_rows(row, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
-                                n_row, d_sino, c->stream),
+                                n_row, d_sino, side, c->stream),
               "radon rows");
     hip_check(hipEventRecord(c->evr[1], c->stream), "event");
     hip_check(launch_radon_rows(col, h, w, nz, na, bins, c->cur.sin, c->cur.cos,
-                                c->cur.alist + n_row, na - n_row, d_sino, c->stream),
+                                c->cur.alist + n_row, na - n_row, d_sino, side, c->stream),
               "radon rows of the transpose");
     hip_check(hipEventRecord(c->evr[2], c->stream), "event");
     c->radon_timed = true;
@@ -423,8 +426,8 @@ size_t ct_hip_bytes_in_use(const ct_hip_ctx* c) { return c ? c->in_use : 0; }
 int ct_hip_clear_workspace(ct_hip_ctx* c) {
     return guarded(c, [&] {
         hip_check(hipStreamSynchronize(c->stream), "sync");
-        for (Buf* b : {&c->vol, &c->row, &c->col, &c->sino, &c->filt, &c->out, &c->keys, &c->mm,
-                       &c->ell, &c->pc_cnt, &c->pc_pts, &c->pc_coord, &c->met})
+        for (Buf* b : {&c->vol, &c->row, &c->col, &c->side, &c->sino, &c->filt, &c->out,
+                       &c->keys, &c->mm, &c->ell, &c->pc_cnt, &c->pc_pts, &c->pc_coord, &c->met})
             release(c, *b);
         for (auto& kv : c->trig) {
             dfree(c, kv.second.cos);

@@ -107,10 +107,12 @@ hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const floa
 // Radon of the n angles alist[0..n) over an image copy [zb][h][w + kPadX]
 // walked by rows, |sin| <= |cos| for every angle: the LDS-staged row walk.
 // Steeper angles run it on the column copy with (w, h) and (d_cos, d_sin)
-// exchanged.
+// exchanged. d_side holds radon_side_floats(w, bins, nz, na) floats (the
+// sums that cross bin blocks; none when every angle fits one block).
+size_t radon_side_floats(int w, int bins, int nz, int na);
 hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, int bins,
                              const float* d_cos, const float* d_sin, const int* d_alist, int n,
-                             float* d_sino, hipStream_t s);
+                             float* d_sino, float* d_side, hipStream_t s);
 
 // ---- row filter ----------------------------------------------------------
 struct FiltArgs {

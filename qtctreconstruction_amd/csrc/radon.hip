@@ -314,15 +314,10 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
     return g;
 }
 
-// Intervals per bin block: a block of NT lanes takes the intervals
-// [j(NT-1), j(NT-1) + NT) and writes the bins of its lanes 1.. (lane 0 too
-// in the first block), since bin i needs lane i-1's B. Neighbouring blocks
-// share one interval.
-__host__ __device__ inline int radon_bin_blocks(int bins, int NT) {
-    const int b = (bins - 1 + NT - 2) / (NT - 1);
-    return b < 1 ? 1 : b;
-}
-
+// A block of NT lanes takes the intervals [j NT, (j+1) NT) of one angle and
+// z-block. Bin i needs lane i-1's B: within a block it is read from LDS; the
+// block's last lane leaves its B in side[(z na + a) (bb-1) + j], and
+// radon_seam_kernel adds it to bin (j+1) NT once the walk is done.
 // Row walk over a w x h image copy [zb][h][w + kPadX] for angles with
 // |sin| <= |cos|: every lane of a block (NT intervals of one angle and
 // z-block) walks all rows top to bottom with one chunk of U = 2 candidates
@@ -353,7 +348,8 @@ template <int RR, int U, int kRowW, int NT, bool FULL>
 __global__ __launch_bounds__(NT) void radon_rows_kernel(
     const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
     const float* __restrict__ cos_t, const float* __restrict__ sin_t,
-    const int* __restrict__ alist, float* __restrict__ sino, int n_ang, int gsz) {
+    const int* __restrict__ alist, float* __restrict__ sino, float* __restrict__ side,
+    int n_ang, int gsz) {
     // window: lanes NT-1 bins apart are at most (NT-1)/|cos| pixels apart,
     // plus a chunk of U; the window must also keep every chunk inside the
     // zero pad
@@ -365,13 +361,14 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
     __shared__ float4 t0[RR][kRowW];  // slices 0-3 of the staged pixels
     __shared__ float4 t1[RR][kRowW];  // slices 4-7
     __shared__ int xw_s[2][RR];       // window starts: staged rows, next rows
-    const int bb = radon_bin_blocks(bins, NT);
+    const int bb = (bins + NT - 1) / NT;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int slots = (bb * n_ang + gsz - 1) / gsz;
     int zb, slot;
     if (!xcd_map(blockIdx.x, gsz, slots, bb * n_ang, nzb, zb, slot)) return;  // whole block
     const int a = alist[slot / bb];
-    const int b0 = (slot % bb) * (NT - 1);
+    const int jb = slot % bb;
+    const int b0 = jb * NT;
     const int i = b0 + threadIdx.x;
     const bool live = i < bins;
     const float c = cos_t[a], sn = sin_t[a];
@@ -454,14 +451,14 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
         }
     }
     // bin i = A_i + B_(i-1): every lane leaves its B in LDS (the staging
-    // buffers, consumed), lane t > 0 adds lane t-1's; lane 0 writes only in
-    // the first bin block, where bin 0 has no B before it
+    // buffers, consumed), lane t > 0 adds lane t-1's; the last lane's B goes
+    // to the side buffer when a next bin block exists
     __syncthreads();
     float4* bx = &t0[0][0];
     bx[2 * threadIdx.x] = make_float4(accB[0], accB[1], accB[2], accB[3]);
     bx[2 * threadIdx.x + 1] = make_float4(accB[4], accB[5], accB[6], accB[7]);
     __syncthreads();
-    if (live && (threadIdx.x > 0 || b0 == 0)) {
+    if (live) {
         float pb[kRZ];
 #pragma unroll
         for (int k = 0; k < kRZ; ++k) pb[k] = 0.0f;
@@ -476,12 +473,28 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
             pb[6] = p1.z;
             pb[7] = p1.w;
         }
+        const bool seam = threadIdx.x == NT - 1 && i + 1 < bins;
 #pragma unroll
         for (int k = 0; k < kRZ; ++k) {
             const int z = zb * kRZ + k;
-            if (z < nz) sino[((size_t)z * na + a) * bins + i] = accA[k] + pb[k];
+            if (z < nz) {
+                sino[((size_t)z * na + a) * bins + i] = accA[k] + pb[k];
+                if (seam) side[((size_t)z * na + a) * (bb - 1) + jb] = accB[k];
+            }
         }
     }
+}
+
+// Adds each bin block's last B to the next block's first bin (grid: seams x
+// angles of the walk, z).
+__global__ void radon_seam_kernel(float* __restrict__ sino, const float* __restrict__ side,
+                                  const int* __restrict__ alist, int n_ang, int na, int bins,
+                                  int NT, int nseam) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nseam * n_ang) return;
+    const int j = t % nseam, a = alist[t / nseam];
+    const size_t za = (size_t)blockIdx.y * na + a;
+    sino[za * bins + (size_t)(j + 1) * NT] += side[za * nseam + j];
 }
 
 hipError_t launch_minmax_init(unsigned* d_keys, int nz, hipStream_t s) {
@@ -516,23 +529,35 @@ hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const floa
     return hipGetLastError();
 }
 
+size_t radon_side_floats(int w, int bins, int nz, int na) {
+    const int NT = w + kPadX <= 520 ? 512 : 256;
+    return (size_t)nz * na * ((bins + NT - 1) / NT - 1);
+}
+
 hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, int bins,
                              const float* d_cos, const float* d_sin, const int* d_alist, int n,
-                             float* d_sino, hipStream_t s) {
+                             float* d_sino, float* d_side, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const bool full = w + kPadX <= 520;  // whole rows fit the 512-lane form's window
     const int NT = full ? 512 : 256;
-    const int bb = radon_bin_blocks(bins, NT);
+    const int bb = (bins + NT - 1) / NT;
     const int gsz = xcd_group(bb * n, nzb);
     const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
     const dim3 grid((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
     if (full)
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 520, 512, true>), grid, dim3(512), 0,
-                           s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n, gsz);
+                           s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
+                           n, gsz);
     else
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 368, 256, false>), grid, dim3(256),
-                           0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, n, gsz);
+                           0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
+                           n, gsz);
+    if (bb > 1) {
+        const int nseam = bb - 1;
+        hipLaunchKernelGGL(radon_seam_kernel, dim3((nseam * n + 255) / 256, nz), dim3(256), 0, s,
+                           d_sino, d_side, d_alist, n, na, bins, NT, nseam);
+    }
     return hipGetLastError();
 }
 

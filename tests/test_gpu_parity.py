@@ -184,13 +184,26 @@ def test_volume_c1_config(backend, O):
     assert err <= TOL
 
 
-@pytest.mark.parametrize("d", [1, 3, 5, 6])
+@pytest.mark.parametrize("d", [1, 3, 5, 6, 13, 20, 41])
 def test_volume_ragged_depth(backend, O, d):
     vol = O.phantom(d, 8, 40, 40, d)
     params = ct.ReconstructionParams(ct.FilterType.Hann, 36)
     out = backend.reconstructVolume(vol, params)
     ref, _ = O.reconstruct_volume(vol, 36, O.HANN)
     assert O.rel_l2(out, ref) <= TOL
+
+
+@pytest.mark.parametrize("n,d,na", [(48, 20, 30), (64, 41, 24), (600, 17, 6)])
+def test_volume_sinograms_every_slice(backend, O, n, d, na):
+    """The volume Radon stage (8-slice z-blocks, an odd count among them,
+    the last one partial) against Serial slice by slice; 600 px rows take
+    the windowed walk."""
+    vol = O.phantom(n + d, 10, n, n, d)
+    _, sino = backend.reconstructVolume(vol, ct.ReconstructionParams(ct.FilterType.Ramp, na),
+                                        want_sinograms=True)
+    for z in range(d):
+        ref, _ = O.compute_sinogram(vol[z], na, n)
+        _sino_parity(O, sino[z], ref)
 
 
 def test_volume_non_square_slices_stay_zero(backend, O):
