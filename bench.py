@@ -29,18 +29,21 @@ METRIC = "FBP voxel-updates/s + p50/p99 latency, served vs plain HIP, relL2≤1e
 PEAK_FP32_TFLOPS = 157.3   # MI355X FP32 (vector and matrix alike)
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 FLOP_PER_UPDATE = 9        # backprojection: t (3), u (1), frac (1), lerp (3), accumulate (1)
-# Radon: per (pixel, angle, slice) the two deposits v*(1-frac), v*frac and their
-# two accumulations (SerialBackend.cpp:113-120); the geometry (s, floor, frac)
-# is per (pixel, angle) and shared by the slices of a z-block.
-FLOP_PER_SAMPLE = 4
+# Radon, per (pixel, angle) deposit of one slice: the same count as an update
+# (s (3+1), frac (1), the two weights and products (3), the accumulations (1));
+# SerialBackend.cpp:110-120 spends it on every pixel of every slice.
+FLOP_PER_SAMPLE = 9
+# of which the deposit arithmetic alone (two products, two sums): the rest is
+# geometry, which the walk computes once per pixel and angle for 8 slices
+FLOP_PER_SAMPLE_DEPOSIT = 4
 # HBM bytes per launch of each kernel at the 512^3 x 720 workload, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over one step, each scaled
 # by the counter's reading for a kernel of known traffic (minmax_kernel reads
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows_kernel": 1.1133e+10,
-    "bp_kernel": 6.458e+09,
+    "radon_rows_kernel": 9.644e+09,
+    "bp_kernel": 6.459e+09,
 }
 
 
@@ -94,13 +97,18 @@ def dominant_roofline(n, na, stage, radon_k, launches):
     per_launch_ms = t_ms / nl
     achieved = flop / (t_ms / 1e3) / 1e12
     tb = TRAFFIC_BYTES.get(name)
-    return {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": tb, "kernel": name,
-            "launches_per_step": nl, "avg_launch_ms": per_launch_ms,
-            "note": f"{what}; FP32 VALU class (no MFMA work on this path); "
-                    f"{'4 FLOP per pixel-angle-slice' if name.startswith('radon') else '9 FLOP per voxel-update'}"
-                    "; 157.3 TF is MI355X's FP32 peak (packed FP32 counted), the kernels are "
-                    "built without packed FP32 ops"}
+    out = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+           "frac": achieved / PEAK_FP32_TFLOPS, "traffic": tb, "kernel": name,
+           "launches_per_step": nl, "avg_launch_ms": per_launch_ms,
+           "note": f"{what}; FP32 VALU class (no MFMA work on this path); "
+                   f"{'9 FLOP per pixel-angle-slice deposit' if name.startswith('radon') else '9 FLOP per voxel-update'}"
+                   "; 157.3 TF is MI355X's FP32 peak (packed FP32 counted), the kernels are "
+                   "built without packed FP32 ops"}
+    if name.startswith("radon"):
+        dep = FLOP_PER_SAMPLE_DEPOSIT * samples * na / (t_ms / 1e3) / 1e12
+        out["deposit_arithmetic_only"] = {"flop_per_sample": FLOP_PER_SAMPLE_DEPOSIT,
+                                          "achieved": dep, "frac": dep / PEAK_FP32_TFLOPS}
+    return out
 
 
 def precision_form(ctx, d_sino, n, na, sample_z, host_sinos_sample):
