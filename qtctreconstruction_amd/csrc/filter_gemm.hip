@@ -24,7 +24,10 @@
 //
 // Work split: a block owns 16 angles x 4 slices of one z-block (4 MFMA row
 // groups, one per slice) and 256 output bins; each of its 4 waves owns 64 bins
-// (4 MFMA 16x16 tiles). MFMA shapes: v_mfma_f32_16x16x32_bf16 and
+// (4 MFMA 16x16 tiles); a launch block holds up to 3 such column blocks. The
+// A operand (the rows' differences, rounded to the variant's format) is
+// converted once per block and K-chunk into LDS and read by every wave; the
+// B fragments come from T in L2. MFMA shapes: v_mfma_f32_16x16x32_bf16 and
 // v_mfma_f32_16x16x32_fp8_fp8. Per lane l (c = l%16, g = l/16): A holds row c,
 // k = 8g..8g+7; B holds column c, k = 8g..8g+7; D holds rows 4g..4g+3 of
 // column c. Because the 4 slice tiles of a (row, column) land in the same
@@ -163,82 +166,154 @@ __global__ void toeplitz_fp8_kernel(const float* __restrict__ T32, size_t n,
     if (i + 1 < n) T8[i + 1] = (unsigned char)((p >> 8) & 0xff);
 }
 
+// The A operand of one K-chunk (32 dp values of each of the block's 4 slices
+// x 16 angles) is converted once per block into LDS: wave w converts slice
+// w, lane (c, g) the fragment row c, k = 8g..8g+7 that lane (c, g) of every
+// wave reads back. Rows are kArow uint4 apart (4 fragments + 1 of padding:
+// 80 bytes, so the 16 lanes of a fragment column land on distinct b128 slots,
+// 5c + g mod 16). FP8 fragments use the first 8 bytes of a slot.
+constexpr int kArow = 5;
+
+// p[kb-1 .. kb+7] of one row (zeros outside [0, bins) and for rows past the
+// volume)
+__device__ __forceinline__ void load_a_row(const float* __restrict__ prow, bool live, bool vec,
+                                           int bins, int kb, float (&pv)[9]) {
+    if (live && vec && kb + 8 <= bins) {
+        const float4 q0 = *reinterpret_cast<const float4*>(prow + kb);
+        const float4 q1 = *reinterpret_cast<const float4*>(prow + kb + 4);
+        pv[0] = kb > 0 ? prow[kb - 1] : 0.0f;
+        pv[1] = q0.x;
+        pv[2] = q0.y;
+        pv[3] = q0.z;
+        pv[4] = q0.w;
+        pv[5] = q1.x;
+        pv[6] = q1.y;
+        pv[7] = q1.z;
+        pv[8] = q1.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int k = kb - 1 + j;
+            pv[j] = (live && k >= 0 && k < bins) ? prow[k] : 0.0f;
+        }
+    }
+}
+
+// the fragment dp[kb .. kb+7] in the variant's format
 template <int VARIANT>
-__global__ __launch_bounds__(256) void toeplitz_gemm_kernel(
+__device__ __forceinline__ void convert_a(const float (&pv)[9], float rsc, uint4& hi, uint4& lo) {
+    float x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = pv[j + 1] - pv[j];  // dp[kb+j]
+    if (VARIANT == 3) {
+        int l = 0, h = 0;
+        l = __builtin_amdgcn_cvt_pk_fp8_f32(x[0] * rsc, x[1] * rsc, l, false);
+        l = __builtin_amdgcn_cvt_pk_fp8_f32(x[2] * rsc, x[3] * rsc, l, true);
+        h = __builtin_amdgcn_cvt_pk_fp8_f32(x[4] * rsc, x[5] * rsc, h, false);
+        h = __builtin_amdgcn_cvt_pk_fp8_f32(x[6] * rsc, x[7] * rsc, h, true);
+        hi = make_uint4((unsigned)l, (unsigned)h, 0u, 0u);
+    } else {
+        // round to nearest even (v_cvt_pk_bf16_f32); the low part is the
+        // rounded remainder
+        bf16x8 ah, al;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            ah[j] = (__bf16)x[j];
+            if (VARIANT == 2) al[j] = (__bf16)(x[j] - (float)ah[j]);
+        }
+        hi = __builtin_bit_cast(uint4, ah);
+        if (VARIANT == 2) lo = __builtin_bit_cast(uint4, al);
+    }
+}
+
+// B fragments of the wave's 4 column tiles for the chunk at kb
+template <int VARIANT>
+__device__ __forceinline__ void load_b(const unsigned short* __restrict__ Thi,
+                                       const unsigned short* __restrict__ Tlo,
+                                       const unsigned char* __restrict__ T8, int n0, int c, int Kp,
+                                       int kb, bf16x8 (&bh)[4], bf16x8 (&bl)[4], long (&b8)[4]) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const size_t off = (size_t)(n0 + 16 * t + c) * Kp + kb;
+        if (VARIANT == 3) {
+            b8[t] = *reinterpret_cast<const long*>(T8 + off);
+        } else {
+            bh[t] = *reinterpret_cast<const bf16x8*>(Thi + off);
+            if (VARIANT == 2) bl[t] = *reinterpret_cast<const bf16x8*>(Tlo + off);
+        }
+    }
+}
+
+// grid: kXcds * ngrp * ceil(nmb / kXcds) blocks of 256 nbw threads; nmb row
+// blocks (z-block, 16 angles), ngrp groups of nbw column blocks of 256 output
+// bins (nbw divides Np / 256). Every wave owns 64 output bins; waves 0-3 also
+// convert the A chunk that all of them read. A row block's column groups run
+// one after the other on one XCD (blocks kXcds apart share an XCD), so its
+// sinogram rows are read from HBM once.
+template <int VARIANT>
+__global__ __launch_bounds__(768) void toeplitz_gemm_kernel(
     const float* __restrict__ sino, int nz, int na, int bins, int sq, int Kp,
     const unsigned short* __restrict__ Thi, const unsigned short* __restrict__ Tlo,
     const unsigned char* __restrict__ T8, const float* __restrict__ rmax,
-    const float* __restrict__ tmax, float4* __restrict__ filt) {
+    const float* __restrict__ tmax, float4* __restrict__ filt, int ngrp, int nmb, int vec) {
+    const int nbw = blockDim.x / 256;
+    const int xcd = blockIdx.x % kXcds, rr = blockIdx.x / kXcds;
+    const int cg = rr % ngrp, mb = (rr / ngrp) * kXcds + xcd;
+    if (mb >= nmb) return;  // whole block
+    const int nab = (na + 15) / 16;
+    const int zb = mb / nab, a0 = (mb % nab) * 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int c = lane & 15, g = lane >> 4;
-    const int n0 = blockIdx.x * 256 + wave * 64;  // first output bin of this wave
-    const int a0 = blockIdx.y * 16;
-    const int zb = blockIdx.z;
-    const int a = a0 + c;  // A-operand row of this lane
+    const int n0 = cg * nbw * 256 + wave * 64;  // first output bin of this wave
+    const bool stager = wave < kZB;
+    // staging role (waves 0-3): slice `wave`, angle a0 + c
+    const int zs = zb * kZB + wave, as = a0 + c;
+    const bool slive = stager && zs < nz && as < na;
+    const float* prow = sino + (slive ? ((size_t)zs * na + as) * bins : 0);
+    float rsc = 1.0f;
+    if (VARIANT == 3 && slive) {
+        const float m = rmax[(size_t)zs * na + as];
+        rsc = (m > 0.0f) ? 448.0f / m : 1.0f;
+    }
+    __shared__ uint4 Ah[2][kZB][16][kArow];
+    __shared__ uint4 Al[VARIANT == 2 ? 2 : 1][kZB][16][kArow];
+
     f4v acc[kZB][4];
 #pragma unroll
     for (int s = 0; s < kZB; ++s)
 #pragma unroll
         for (int t = 0; t < 4; ++t) acc[s][t] = (f4v){0.f, 0.f, 0.f, 0.f};
 
-    const float* rowp[kZB];
-    float rs[kZB];
-#pragma unroll
-    for (int s = 0; s < kZB; ++s) {
-        const int z = zb * kZB + s;
-        const bool live = (z < nz) && (a < na);
-        rowp[s] = live ? sino + ((size_t)z * na + a) * bins : nullptr;
-        rs[s] = 1.0f;
-        if (VARIANT == 3 && live) {
-            const float m = rmax[(size_t)z * na + a];
-            rs[s] = (m > 0.0f) ? 448.0f / m : 1.0f;
+    const int nch = Kp / 32;
+    for (int kc = 0; kc < nch; ++kc) {
+        const int buf = kc & 1;
+        const int kb = kc * 32 + 8 * g;
+        if (stager) {
+            float pv[9];
+            uint4 h, l;
+            load_a_row(prow, slive, vec != 0, bins, kb, pv);
+            convert_a<VARIANT>(pv, rsc, h, l);
+            Ah[buf][wave][c][g] = h;
+            if (VARIANT == 2) Al[buf][wave][c][g] = l;
         }
-    }
-
-    for (int k0 = 0; k0 < Kp; k0 += 32) {
-        const int kb = k0 + 8 * g;
-        // B fragments for the wave's 4 column tiles
         bf16x8 bh[4], bl[4];
         long b8[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const size_t off = (size_t)(n0 + 16 * t + c) * Kp + kb;
-            if (VARIANT == 3) {
-                b8[t] = *reinterpret_cast<const long*>(T8 + off);
-            } else {
-                bh[t] = *reinterpret_cast<const bf16x8*>(Thi + off);
-                if (VARIANT == 2) bl[t] = *reinterpret_cast<const bf16x8*>(Tlo + off);
-            }
-        }
+        load_b<VARIANT>(Thi, Tlo, T8, n0, c, Kp, kb, bh, bl, b8);
+        // the chunk is staged; the buffer written next was last read before
+        // this barrier
+        __syncthreads();
 #pragma unroll
         for (int s = 0; s < kZB; ++s) {
-            float pv[9];  // p[kb-1 .. kb+7]
-#pragma unroll
-            for (int j = 0; j < 9; ++j) {
-                const int k = kb - 1 + j;
-                pv[j] = (rowp[s] && k >= 0 && k < bins) ? rowp[s][k] : 0.0f;
-            }
-            float x[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j) x[j] = pv[j + 1] - pv[j];  // dp[kb+j]
             if (VARIANT == 3) {
-                int lo = 0, hi = 0;
-                lo = __builtin_amdgcn_cvt_pk_fp8_f32(x[0] * rs[s], x[1] * rs[s], lo, false);
-                lo = __builtin_amdgcn_cvt_pk_fp8_f32(x[2] * rs[s], x[3] * rs[s], lo, true);
-                hi = __builtin_amdgcn_cvt_pk_fp8_f32(x[4] * rs[s], x[5] * rs[s], hi, false);
-                hi = __builtin_amdgcn_cvt_pk_fp8_f32(x[6] * rs[s], x[7] * rs[s], hi, true);
-                const long av = (long)(unsigned)lo | ((long)(unsigned)hi << 32);
+                const uint4 q = Ah[buf][s][c][g];
+                const long av = (long)q.x | ((long)q.y << 32);
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
                     acc[s][t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(av, b8[t], acc[s][t], 0, 0, 0);
             } else {
-                bf16x8 ah, al;
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const unsigned short hb = f2bf(x[j]);
-                    ah[j] = __builtin_bit_cast(__bf16, hb);
-                    if (VARIANT == 2) al[j] = __builtin_bit_cast(__bf16, f2bf(x[j] - bf2f(hb)));
-                }
+                const bf16x8 ah = __builtin_bit_cast(bf16x8, Ah[buf][s][c][g]);
+                bf16x8 al;
+                if (VARIANT == 2) al = __builtin_bit_cast(bf16x8, Al[buf][s][c][g]);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     acc[s][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[t], acc[s][t], 0, 0, 0);
@@ -311,19 +386,23 @@ hipError_t launch_toeplitz_gemm(int variant, const float* d_sino, int nz, int na
                                 const float* d_rmax, const float* d_tmax, float4* d_filt,
                                 hipStream_t s) {
     const int nzb = (nz + kZB - 1) / kZB;
-    dim3 grid(Np / 256, (na + 15) / 16, nzb);
+    const int nnb = Np / 256, nmb = nzb * ((na + 15) / 16);
+    const int nbw = nnb % 3 == 0 ? 3 : (nnb % 2 == 0 ? 2 : 1);  // column blocks per block
+    const int ngrp = nnb / nbw;
+    const dim3 grid((unsigned)(kXcds * ngrp * ((nmb + kXcds - 1) / kXcds))), block(256 * nbw);
+    const int vec = (bins % 4 == 0) && ((uintptr_t)d_sino % 16 == 0);
     switch (variant) {
         case 1:
-            hipLaunchKernelGGL(toeplitz_gemm_kernel<1>, grid, dim3(256), 0, s, d_sino, nz, na, bins,
-                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt);
+            hipLaunchKernelGGL(toeplitz_gemm_kernel<1>, grid, block, 0, s, d_sino, nz, na, bins,
+                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt, ngrp, nmb, vec);
             break;
         case 2:
-            hipLaunchKernelGGL(toeplitz_gemm_kernel<2>, grid, dim3(256), 0, s, d_sino, nz, na, bins,
-                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt);
+            hipLaunchKernelGGL(toeplitz_gemm_kernel<2>, grid, block, 0, s, d_sino, nz, na, bins,
+                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt, ngrp, nmb, vec);
             break;
         case 3:
-            hipLaunchKernelGGL(toeplitz_gemm_kernel<3>, grid, dim3(256), 0, s, d_sino, nz, na, bins,
-                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt);
+            hipLaunchKernelGGL(toeplitz_gemm_kernel<3>, grid, block, 0, s, d_sino, nz, na, bins,
+                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt, ngrp, nmb, vec);
             break;
         default:
             return hipErrorInvalidValue;
