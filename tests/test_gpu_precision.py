@@ -36,8 +36,14 @@ def ctx():
 @pytest.mark.parametrize("fb", list(ct.FilterBackend))
 @pytest.mark.parametrize("ftype,shape", [(ct.FilterType.Ramp, (5, 40, 128)),
                                          (ct.FilterType.Hann, (4, 33, 100)),
-                                         (ct.FilterType.SheppLogan, (2, 16, 512))])
+                                         (ct.FilterType.SheppLogan, (2, 16, 512)),
+                                         (ct.FilterType.Ramp, (3, 20, 300)),
+                                         (ct.FilterType.Hann, (2, 12, 600)),
+                                         (ct.FilterType.SheppLogan, (1, 7, 800))])
 def test_filter_variants_vs_fp64(ctx, O, fb, ftype, shape):
+    """Detector widths whose padded outputs take 1 to 5 column blocks of 256
+    bins (GEMM launch blocks of 1, 2 or 3 of them), angle counts that leave a
+    partial group of 16, and partial z-blocks."""
     rng = np.random.default_rng(int(fb) * 10 + int(ftype))
     vol = O.phantom(int(ftype) + 3, 10, shape[2], shape[2], shape[0])
     sinos = np.stack([O.compute_sinogram(vol[z], shape[1], shape[2])[0] for z in range(shape[0])])
