@@ -250,12 +250,25 @@ __device__ __forceinline__ void load_b(const unsigned short* __restrict__ Thi,
 // convert the A chunk that all of them read. A row block's column groups run
 // one after the other on one XCD (blocks kXcds apart share an XCD), so its
 // sinogram rows are read from HBM once.
-template <int VARIANT>
+//
+// BLDS: the B fragments come from LDS instead of T in L2. T is Toeplitz
+// within n < sq, k <= bins (T[n][k] depends on k - n alone); outside it the
+// products do not count (dp[k] = 0 for k > bins, columns n >= sq are not
+// written), so any finite value serves there. The block's columns
+// [nb0, nb0 + NB) and every k give k - n = i - (nb0 + NB - 1), i in [0, Lq),
+// Lq = Kp + NB. The block copies the diagonal Q[i] from T (at n = max(0,
+// -J), k = max(0, J), the same rounded values; 0 where no (n, k) of T's
+// Toeplitz range has this diagonal) and expands it into the window table
+// W[i] = Q[i .. i+7], one 16-byte (FP8: 8-byte) slot per start: lane (c, g)
+// of column tile t reads slot kb - (n - nb0) + NB - 1, so the 16 lanes of a
+// fragment column read 16 consecutive slots (no bank conflicts).
+template <int VARIANT, bool BLDS>
 __global__ __launch_bounds__(768) void toeplitz_gemm_kernel(
     const float* __restrict__ sino, int nz, int na, int bins, int sq, int Kp,
     const unsigned short* __restrict__ Thi, const unsigned short* __restrict__ Tlo,
     const unsigned char* __restrict__ T8, const float* __restrict__ rmax,
     const float* __restrict__ tmax, float4* __restrict__ filt, int ngrp, int nmb, int vec) {
+    extern __shared__ uint4 Wdyn[];
     const int nbw = blockDim.x / 256;
     const int xcd = blockIdx.x % kXcds, rr = blockIdx.x / kXcds;
     const int cg = rr % ngrp, mb = (rr / ngrp) * kXcds + xcd;
@@ -277,6 +290,48 @@ __global__ __launch_bounds__(768) void toeplitz_gemm_kernel(
     }
     __shared__ uint4 Ah[2][kZB][16][kArow];
     __shared__ uint4 Al[VARIANT == 2 ? 2 : 1][kZB][16][kArow];
+    const int NB = nbw * 256, nb0 = cg * NB, Lq = Kp + NB;
+    if (BLDS) {
+        // the diagonal, staged in the A buffers (not yet in use): hi in Ah,
+        // lo in Al, FP8 bytes in Ah
+        unsigned short* qh = reinterpret_cast<unsigned short*>(&Ah[0][0][0][0]);
+        unsigned short* ql = reinterpret_cast<unsigned short*>(&Al[0][0][0][0]);
+        unsigned char* q8 = reinterpret_cast<unsigned char*>(&Ah[0][0][0][0]);
+        for (int i = threadIdx.x; i < Lq + 8; i += blockDim.x) {
+            const int J = i - (nb0 + NB - 1);
+            const bool in = J > -sq && J <= bins;
+            const size_t at = in ? (size_t)(J < 0 ? -J : 0) * Kp + (J > 0 ? J : 0) : 0;
+            if (VARIANT == 3) {
+                q8[i] = in ? T8[at] : (unsigned char)0;
+            } else {
+                qh[i] = in ? Thi[at] : (unsigned short)0;
+                if (VARIANT == 2) ql[i] = in ? Tlo[at] : (unsigned short)0;
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < Lq; i += blockDim.x) {
+            if (VARIANT == 3) {
+                unsigned w0 = 0, w1 = 0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    w0 |= (unsigned)q8[i + e] << (8 * e);
+                    w1 |= (unsigned)q8[i + 4 + e] << (8 * e);
+                }
+                reinterpret_cast<uint2*>(Wdyn)[i] = make_uint2(w0, w1);
+            } else {
+                unsigned wh[4], wl[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    wh[e] = (unsigned)qh[i + 2 * e] | ((unsigned)qh[i + 2 * e + 1] << 16);
+                    if (VARIANT == 2)
+                        wl[e] = (unsigned)ql[i + 2 * e] | ((unsigned)ql[i + 2 * e + 1] << 16);
+                }
+                Wdyn[i] = make_uint4(wh[0], wh[1], wh[2], wh[3]);
+                if (VARIANT == 2) Wdyn[Lq + i] = make_uint4(wl[0], wl[1], wl[2], wl[3]);
+            }
+        }
+        __syncthreads();  // the A buffers are free again
+    }
 
     f4v acc[kZB][4];
 #pragma unroll
@@ -298,7 +353,21 @@ __global__ __launch_bounds__(768) void toeplitz_gemm_kernel(
         }
         bf16x8 bh[4], bl[4];
         long b8[4];
-        load_b<VARIANT>(Thi, Tlo, T8, n0, c, Kp, kb, bh, bl, b8);
+        if (BLDS) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int i0 = kb - (n0 - nb0 + 16 * t + c) + NB - 1;
+                if (VARIANT == 3) {
+                    const uint2 q = reinterpret_cast<const uint2*>(Wdyn)[i0];
+                    b8[t] = (long)q.x | ((long)q.y << 32);
+                } else {
+                    bh[t] = __builtin_bit_cast(bf16x8, Wdyn[i0]);
+                    if (VARIANT == 2) bl[t] = __builtin_bit_cast(bf16x8, Wdyn[Lq + i0]);
+                }
+            }
+        } else {
+            load_b<VARIANT>(Thi, Tlo, T8, n0, c, Kp, kb, bh, bl, b8);
+        }
         // the chunk is staged; the buffer written next was last read before
         // this barrier
         __syncthreads();
@@ -391,22 +460,35 @@ hipError_t launch_toeplitz_gemm(int variant, const float* d_sino, int nz, int na
     const int ngrp = nnb / nbw;
     const dim3 grid((unsigned)(kXcds * ngrp * ((nmb + kXcds - 1) / kXcds))), block(256 * nbw);
     const int vec = (bins % 4 == 0) && ((uintptr_t)d_sino % 16 == 0);
+    // window table of the B fragments: Lq slots of 16 bytes per bf16 part (8
+    // for FP8), beside the 20 KB of A buffers, within 64 KB of LDS
+    const size_t Lq = (size_t)Kp + 256 * nbw;
+    const size_t wbytes = Lq * (variant == 2 ? 32 : (variant == 1 ? 16 : 8));
+    const bool blds = wbytes <= 44 * 1024;
+    const size_t dyn = blds ? wbytes : 0;
+#define CT_GEMM_LAUNCH(V)                                                                       \
+    if (blds)                                                                                   \
+        hipLaunchKernelGGL((toeplitz_gemm_kernel<V, true>), grid, block, dyn, s, d_sino, nz, na, \
+                           bins, sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt, ngrp, nmb,  \
+                           vec);                                                                \
+    else                                                                                        \
+        hipLaunchKernelGGL((toeplitz_gemm_kernel<V, false>), grid, block, 0, s, d_sino, nz, na,  \
+                           bins, sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt, ngrp, nmb,  \
+                           vec);
     switch (variant) {
         case 1:
-            hipLaunchKernelGGL(toeplitz_gemm_kernel<1>, grid, block, 0, s, d_sino, nz, na, bins,
-                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt, ngrp, nmb, vec);
+            CT_GEMM_LAUNCH(1)
             break;
         case 2:
-            hipLaunchKernelGGL(toeplitz_gemm_kernel<2>, grid, block, 0, s, d_sino, nz, na, bins,
-                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt, ngrp, nmb, vec);
+            CT_GEMM_LAUNCH(2)
             break;
         case 3:
-            hipLaunchKernelGGL(toeplitz_gemm_kernel<3>, grid, block, 0, s, d_sino, nz, na, bins,
-                               sq, Kp, d_Thi, d_Tlo, d_T8, d_rmax, d_tmax, d_filt, ngrp, nmb, vec);
+            CT_GEMM_LAUNCH(3)
             break;
         default:
             return hipErrorInvalidValue;
     }
+#undef CT_GEMM_LAUNCH
     return hipGetLastError();
 }
 
