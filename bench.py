@@ -262,6 +262,23 @@ def served_form(ctx, n_jobs, seed=5, pool_gb=96.0, rate_factor=0.7):
     t_plain = time.perf_counter() - t0
     plain_ups = float(upd[: len(sample)].sum()) / t_plain
     d_out.free()
+    # The first 12 of those jobs through the host-buffer entry, as a caller of
+    # reconstructVolumeFromSinograms hands them over (pageable host sinograms
+    # in, host volume out): the plain per-job rate with its PCIe transfers.
+    f32p = ctypes.POINTER(ctypes.c_float)
+    hsample = sample[:12]
+    t_host = 0.0
+    for (N, d, na, f) in hsample:
+        hs = pool[(N, na)].download((d, na, N))
+        ho = np.empty((d, N, N), np.float32)
+        t0 = time.perf_counter()
+        ctx._call("ct_hip_reconstruct_from_sinograms", ctx.handle, hs.ctypes.data_as(f32p), d, na,
+                  N, angles[na].ctypes.data_as(f32p), f, ho.ctypes.data_as(f32p))
+        t_host += time.perf_counter() - t0
+        del hs, ho
+    host_ups = float(sum(N * N * d * na for (N, d, na, f) in hsample)) / t_host
+    log(f"[bench] serving: plain through host buffers {host_ups:.3e} upd/s "
+        f"({len(hsample)} jobs, PCIe included)")
     mean_job_s = t_plain / len(sample)
     lam = rate_factor / mean_job_s
     log(f"[bench] serving: plain {plain_ups:.3e} upd/s, {1/mean_job_s:.1f} jobs/s -> rate {lam:.1f}/s")
@@ -310,6 +327,12 @@ def served_form(ctx, n_jobs, seed=5, pool_gb=96.0, rate_factor=0.7):
     for d_s in pool.values():
         d_s.free()
     return {"jobs": n_jobs, "rate_per_s": lam, "rate_factor_of_plain": rate_factor,
+            "plain_per_job_host_buffers_voxel_updates_per_s": host_ups,
+            "poisson_vs_plain_device": poisson["voxel_updates_per_s"] / plain_ups,
+            "poisson_vs_plain_host_buffers": poisson["voxel_updates_per_s"] / host_ups,
+            "plain_host_buffers_note": f"first {len(hsample)} jobs of the mix through "
+                                       "ct_hip_reconstruct_from_sinograms (pageable host "
+                                       "sinograms in, host volume out, PCIe included)",
             "plain_per_job_voxel_updates_per_s": plain_ups, "saturated": saturated,
             "poisson": poisson, "poisson_fifo": poisson_fifo,
             "policy": "size-ordered with waiting credit 0.25, batch cap max(5 ms, queue/4); "
