@@ -40,12 +40,13 @@ def ctx():
                                          (ct.FilterType.Ramp, (3, 20, 300)),
                                          (ct.FilterType.Hann, (2, 12, 600)),
                                          (ct.FilterType.SheppLogan, (1, 7, 800)),
-                                         (ct.FilterType.Hann, (1, 5, 1000))])
+                                         (ct.FilterType.Hann, (1, 5, 1000)),
+                                         (ct.FilterType.Ramp, (1, 3, 1086))])
 def test_filter_variants_vs_fp64(ctx, O, fb, ftype, shape):
     """Detector widths whose padded outputs take 1 to 6 column blocks of 256
-    bins (GEMM launch blocks of 1, 2 or 3 of them; at 1000 bins the split
-    form's window table exceeds its LDS share and the B fragments come from
-    T in L2), angle counts that leave a partial group of 16, and partial
+    bins (GEMM launch blocks of 1, 2 or 3 of them; at 1000 and 1086 bins the
+    split form's window table exceeds its LDS share and the B fragments come
+    from T in L2), angle counts that leave a partial group of 16, and partial
     z-blocks."""
     rng = np.random.default_rng(int(fb) * 10 + int(ftype))
     vol = O.phantom(int(ftype) + 3, 10, shape[2], shape[2], shape[0])
