@@ -6,10 +6,12 @@
 // the host restatement: same float operations in the same order, contraction
 // off (the library is built with -ffp-contract=off).
 //
-// Each block covers a 64x4 (x,y) tile of one z slice. Ellipsoids whose
-// bounding box misses the tile are skipped with a wave-uniform test; skipping
-// is exact because a voxel outside the box cannot pass the membership test
-// (the box carries a 1e-3 margin, far above the float error of the form).
+// Each block covers a 64x4 (x,y) tile of kPhZ = 4 z slices. Ellipsoids whose
+// bounding box misses the tile are skipped; skipping is exact because a voxel
+// outside the box cannot pass the membership test (the box carries a 1e-3
+// margin, far above the float error of the form). The box tests run one
+// ellipsoid per lane and a ballot gives the wave the ones to evaluate, so
+// the test's loads are not serialised per ellipsoid.
 #include "ct_internal.h"
 
 namespace cthip {
@@ -19,39 +21,71 @@ __device__ __forceinline__ float lin_coord(int i, int n, float step) {
     return (n == 1) ? -1.0f : -1.0f + step * (float)i;
 }
 
+constexpr int kPhZ = 4;  // z slices per thread
+
 __global__ __launch_bounds__(256) void phantom_kernel(const PhEll* __restrict__ e, int n, int nx,
                                                       int ny, int nz, float* __restrict__ vol) {
     const int x0 = blockIdx.x * 64, y0 = blockIdx.y * 4;
     const int x = x0 + (threadIdx.x & 63);
     const int y = y0 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
+    const int z0 = blockIdx.z * kPhZ;
+    const int nzl = min(kPhZ, nz - z0);
     const float stx = 2.0f / (float)(nx - 1);
     const float sty = 2.0f / (float)(ny - 1);
     const float stz = 2.0f / (float)(nz - 1);
     const float xf = lin_coord(x, nx, stx);
     const float yf = lin_coord(y, ny, sty);
-    const float zf = lin_coord(z, nz, stz);
+    float zf[kPhZ];
+#pragma unroll
+    for (int j = 0; j < kPhZ; ++j) zf[j] = lin_coord(min(z0 + j, nz - 1), nz, stz);
     // tile coordinate box (linspace is increasing)
     const float tx0 = lin_coord(x0, nx, stx);
     const float tx1 = lin_coord(min(x0 + 63, nx - 1), nx, stx);
     const float ty0 = lin_coord(y0, ny, sty);
     const float ty1 = lin_coord(min(y0 + 3, ny - 1), ny, sty);
-    float v = 0.0f;
-    for (int k = 0; k < n; ++k) {
-        const PhEll& q = e[k];
-        if (tx1 < q.lo[0] || tx0 > q.hi[0] || ty1 < q.lo[1] || ty0 > q.hi[1] || zf < q.lo[2] ||
-            zf > q.hi[2])
-            continue;
-        const float dx = xf - q.cx;
-        const float dy = yf - q.cy;
-        const float dz = zf - q.cz;
-        const float q0 = q.m[0] * dx + q.m[1] * dy + q.m[2] * dz;
-        const float q1 = q.m[3] * dx + q.m[4] * dy + q.m[5] * dz;
-        const float q2 = q.m[6] * dx + q.m[7] * dy + q.m[8] * dz;
-        const float f = q0 * q0 * q.ia[0] + q1 * q1 * q.ia[1] + q2 * q2 * q.ia[2];
-        if (f <= 1.0f) v += q.rho;
+    const float tz0 = zf[0], tz1 = zf[kPhZ - 1];
+    float v[kPhZ];
+#pragma unroll
+    for (int j = 0; j < kPhZ; ++j) v[j] = 0.0f;
+    // 64 ellipsoids at a time: lane j tests ellipsoid kb + j against the tile
+    // box, and the wave then visits the ones that pass in ascending order
+    // (the additions keep the host's order)
+    for (int kb = 0; kb < n; kb += 64) {
+        const int kj = kb + (int)(threadIdx.x & 63);
+        bool pass = false;
+        if (kj < n) {
+            const PhEll& t = e[kj];
+            pass = !(tx1 < t.lo[0] || tx0 > t.hi[0] || ty1 < t.lo[1] || ty0 > t.hi[1] ||
+                     tz1 < t.lo[2] || tz0 > t.hi[2]);
+        }
+        unsigned long long m = __ballot(pass);
+        while (m) {
+            const int k = kb + __ffsll((long long)m) - 1;
+            m &= m - 1;
+            const PhEll& q = e[k];
+            const float dx = xf - q.cx;
+            const float dy = yf - q.cy;
+            // (m0 dx + m1 dy) is the same float for every slice; the slice's
+            // m2 dz is added last, as in the host's left-to-right sum
+            const float a0 = q.m[0] * dx + q.m[1] * dy;
+            const float a1 = q.m[3] * dx + q.m[4] * dy;
+            const float a2 = q.m[6] * dx + q.m[7] * dy;
+#pragma unroll
+            for (int j = 0; j < kPhZ; ++j) {
+                const float dz = zf[j] - q.cz;
+                const float q0 = a0 + q.m[2] * dz;
+                const float q1 = a1 + q.m[5] * dz;
+                const float q2 = a2 + q.m[8] * dz;
+                const float f = q0 * q0 * q.ia[0] + q1 * q1 * q.ia[1] + q2 * q2 * q.ia[2];
+                if (f <= 1.0f) v[j] += q.rho;
+            }
+        }
     }
-    if (x < nx && y < ny) vol[((size_t)z * ny + y) * nx + x] = v;
+    if (x < nx && y < ny) {
+#pragma unroll
+        for (int j = 0; j < kPhZ; ++j)
+            if (j < nzl) vol[((size_t)(z0 + j) * ny + y) * nx + x] = v[j];
+    }
 }
 
 // Synthetic acquisition of a sinogram p (in place): counts ~ Poisson(I0 e^{-kappa p}),
@@ -88,7 +122,7 @@ hipError_t launch_noisy_log(float* d_p, size_t n, float I0, float kappa, unsigne
 
 hipError_t launch_phantom(const PhEll* d_ell, int n, int nx, int ny, int nz, float* d_vol,
                           hipStream_t s) {
-    dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+    dim3 grid((nx + 63) / 64, (ny + 3) / 4, (nz + kPhZ - 1) / kPhZ);
     hipLaunchKernelGGL(phantom_kernel, grid, dim3(256), 0, s, d_ell, n, nx, ny, nz, d_vol);
     return hipGetLastError();
 }
