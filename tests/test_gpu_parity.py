@@ -60,8 +60,14 @@ def _slice(rng, h, w, zero_frac=0.3):
 @pytest.mark.parametrize("h,w,na,bins", [(64, 64, 180, 64), (97, 97, 33, 97), (50, 64, 45, 70),
                                          (64, 50, 7, 40), (32, 32, 1, 32), (128, 128, 360, 128),
                                          (20, 20, 12, 3), (9, 8, 16, 1), (12, 10, 30, 2),
-                                         (24, 600, 20, 600), (600, 24, 20, 300)])
+                                         (24, 600, 20, 600), (600, 24, 20, 300),
+                                         (40, 772, 30, 772), (12, 770, 16, 1600),
+                                         (40, 1028, 24, 1028), (10, 1100, 10, 1100),
+                                         (1100, 10, 10, 700)])
 def test_sinogram_parity(backend, O, h, w, na, bins):
+    """Row lengths at the edges of each walk form (whole rows staged by 512
+    or 768 lanes up to 516 and 772 pixels; the windowed 256-lane form
+    beyond), with detector widths that need several bin blocks."""
     rng = np.random.default_rng(h * 1000 + na)
     s = _slice(rng, h, w)
     got = backend.computeSinogram(s, na, bins)
@@ -85,7 +91,7 @@ def test_sinogram_phantom_slice_and_constant(backend, O):
 
 @pytest.mark.parametrize("h,w,na,bins", [(97, 97, 180, 97), (64, 50, 90, 70), (50, 64, 45, 40),
                                          (20, 20, 12, 3), (33, 33, 360, 33), (16, 700, 24, 700),
-                                         (700, 16, 24, 100)])
+                                         (700, 16, 24, 100), (8, 1100, 12, 1100)])
 def test_sinogram_indexing_bit_exact(backend, O, h, w, na, bins):
     """Every pixel's bins and fraction are Serial's at every angle: with one
     nonzero pixel in the slice each bin receives at most one deposit, so the
