@@ -48,10 +48,11 @@ __device__ __forceinline__ bool xcd_map(int u, int gsz, int slots, int nslot, in
 // XCDs per unit for nunit units of nslot workgroups. Each round takes
 // ceil(nslot / g) workgroups per XCD and there are ceil(nunit / (kXcds / g))
 // rounds; among the g within 2% of the fewest workgroups per XCD over all
-// rounds (so few units still fill every XCD), the one nearest ~300 per XCD
-// and round (about 1.5 times what an XCD holds at once, measured best for
-// the Radon walks), the smaller g on a tie.
-inline int xcd_group(int nslot, int nunit) {
+// rounds (so few units still fill every XCD), the one nearest `target` per
+// XCD and round (300 measured best for the Radon walks of 256- and 512-lane
+// blocks, about 1.5 times what an XCD holds of them at once), the smaller g
+// on a tie.
+inline int xcd_group(int nslot, int nunit, double target = 300.0) {
     auto load = [&](int g) {
         const long long P = kXcds / g;
         return (double)((nunit + P - 1) / P) * (double)((nslot + g - 1) / g);
@@ -63,7 +64,7 @@ inline int xcd_group(int nslot, int nunit) {
     for (int g = 1; g <= kXcds; g *= 2) {
         if (load(g) > best * 1.02) continue;
         const double per = (double)((nslot + g - 1) / g);
-        const double d = per > 300.0 ? per / 300.0 : 300.0 / per;
+        const double d = per > target ? per / target : target / per;
         if (d < dist) {
             dist = d;
             pick = g;

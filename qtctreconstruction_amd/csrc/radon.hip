@@ -341,11 +341,11 @@ constexpr int kRadonRows = 2;  // rows staged per step
 // Workgroups follow the XCD-grouped order (xcd_map): units are z-blocks,
 // a z-block's slots its (bin block, angle) pairs, n_ang angles. A block has
 // NT lanes (NT intervals). FULL: the image row fits the window (w + kPadX <=
-// kRowW), every block stages whole rows and the window start is 0; NT = 512
-// or 768 then (rows up to 516 or 772 pixels), 8 or 12 waves sharing 2 staged
-// rows: 32 B of staging per lane and row, against 46 B for the windowed
-// 256-lane form of wider rows, whose window must span the lanes' rays at any
-// angle.
+// kRowW), every block stages whole rows and the window start is 0; NT = 512,
+// 768 or 1024 then (rows up to 516, 772 or 1028 pixels), 8-16 waves sharing 2
+// staged rows: 32 B of staging per lane and row, against 46 B for the
+// windowed 256-lane form of wider rows, whose window must span the lanes'
+// rays at any angle.
 template <int RR, int U, int kRowW, int NT, bool FULL>
 __global__ __launch_bounds__(NT) void radon_rows_kernel(
     const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
@@ -531,13 +531,11 @@ hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const floa
     return hipGetLastError();
 }
 
-// Block width of the walk over rows of w pixels: whole-row staging with 512
-// or 768 lanes while the row fits the window (NT + 8 pixels), else the
-// windowed 256-lane form. (A 1024-lane whole-row form measured 3% slower
-// than the windowed one at 1024^3 x 720: one bin block per angle puts each
-// z-block on a single XCD.)
+// Block width of the walk over rows of w pixels: whole-row staging with 512,
+// 768 or 1024 lanes while the row fits the window (NT + 8 pixels), else the
+// windowed 256-lane form.
 static int radon_lanes(int w) {
-    return w + kPadX <= 520 ? 512 : (w + kPadX <= 776 ? 768 : 256);
+    return w + kPadX <= 520 ? 512 : (w + kPadX <= 776 ? 768 : (w + kPadX <= 1032 ? 1024 : 256));
 }
 
 size_t radon_side_floats(int w, int bins, int nz, int na) {
@@ -552,7 +550,10 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int NT = radon_lanes(w);
     const int bb = (bins + NT - 1) / NT;
-    const int gsz = xcd_group(bb * n, nzb);
+    // XCD group: ~300 blocks per XCD and round for 512-lane and windowed
+    // blocks, proportionally fewer for the wider ones (measured: one XCD per
+    // z-block at 512^3, two at 768^3 and 1024^3)
+    const int gsz = xcd_group(bb * n, nzb, NT > 512 ? 300.0 * 512.0 / NT : 300.0);
     const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
     const dim3 grid((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
     if (NT == 512)
@@ -563,6 +564,10 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 776, 768, true>), grid, dim3(768), 0,
                            s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
                            n, gsz);
+    else if (NT == 1024)
+        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 1032, 1024, true>), grid,
+                           dim3(1024), 0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist,
+                           d_sino, d_side, n, gsz);
     else
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 368, 256, false>), grid, dim3(256),
                            0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
