@@ -65,9 +65,9 @@ def _slice(rng, h, w, zero_frac=0.3):
                                          (40, 1028, 24, 1028), (10, 1100, 10, 1100),
                                          (1100, 10, 10, 700)])
 def test_sinogram_parity(backend, O, h, w, na, bins):
-    """Row lengths at the edges of each walk form (whole rows staged by 512
-    or 768 lanes up to 516 and 772 pixels; the windowed 256-lane form
-    beyond), with detector widths that need several bin blocks."""
+    """Row lengths at the edges of each walk form (whole rows staged by 512,
+    768 or 1024 lanes up to 516, 772 and 1028 pixels; the windowed 256-lane
+    form beyond), with detector widths that need several bin blocks."""
     rng = np.random.default_rng(h * 1000 + na)
     s = _slice(rng, h, w)
     got = backend.computeSinogram(s, na, bins)
