@@ -57,7 +57,22 @@ __global__ __launch_bounds__(256) void minmax_kernel(const float* __restrict__ v
     const size_t first = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (vec4) {
         const float4* p4 = reinterpret_cast<const float4*>(p);
-        for (size_t i = first; i < slice_elems / 4; i += stride) {
+        const size_t n4 = slice_elems / 4;
+        size_t i = first;
+        // four loads in flight per thread
+        for (; i + 3 * stride < n4; i += 4 * stride) {
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = p4[i + u * stride];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const unsigned a = f2key(q[u].x), b = f2key(q[u].y), c = f2key(q[u].z),
+                               d = f2key(q[u].w);
+                kmin = min(kmin, min(min(a, b), min(c, d)));
+                kmax = max(kmax, max(max(a, b), max(c, d)));
+            }
+        }
+        for (; i < n4; i += stride) {
             const float4 q = p4[i];
             const unsigned a = f2key(q.x), b = f2key(q.y), c = f2key(q.z), d = f2key(q.w);
             kmin = min(kmin, min(min(a, b), min(c, d)));
