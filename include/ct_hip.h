@@ -116,7 +116,10 @@ int ct_hip_reconstruct_slice(ct_hip_ctx* ctx, const float* sino, int num_angles,
  * SerialBackend.cpp:261-318): per z slice, sinogram with bins = w, FBP onto
  * w x w, then v*eff_span + min. vol and out are (d,h,w); when h != w the
  * output stays zero, as Buffer3D::setSlice drops mismatched slices
- * (CTTypes.h:112-116). sino_out (optional) receives [d][na][w]. */
+ * (CTTypes.h:112-116), and only the sinograms are computed (the w x w
+ * reconstructions Serial computes and then drops are not; the timings then
+ * hold the Radon stage and 0 for the rest). sino_out (optional) receives
+ * [d][na][w]. */
 int ct_hip_reconstruct_volume(ct_hip_ctx* ctx, const float* vol, int w, int h, int d,
                               int num_angles, int filter, float* out, float* sino_out);
 
@@ -355,6 +358,21 @@ int ct_hip_radon_dev(ct_hip_ctx* ctx, const float* d_vol, int w, int h, int d, i
  * p = -ln(max(counts,1)/I0)/kappa. Synthetic input for the serving form. */
 int ct_hip_noisy_log_dev(ct_hip_ctx* ctx, float* d_sino, size_t n, float I0, float kappa,
                          uint64_t seed);
+
+/* Shape limits of the kernels, the same checks every entry point makes before
+ * any allocation or launch; needs no context and no device. No reference
+ * counterpart (the reference's backends accept any shape):
+ *   CT_STAGE_RADON  forward projection of a w x h slice into `bins` bins
+ *                   (computeSinogram, reconstructVolume, the sinogram export):
+ *                   w, h <= 2048 and bins <= 4096, the range over which the
+ *                   walk's candidate margin is proven against Serial's float s;
+ *   CT_STAGE_FBP    filter + backprojection of `bins`-bin rows with the given
+ *                   CT_FILTER_BACKEND_*: the FFT backend needs nfft <= 4096
+ *                   (bins <= 1448); w and h are ignored.
+ * Returns CT_OK, CT_ERR_INVALID (a size <= 0, an unknown stage) or
+ * CT_ERR_UNSUPPORTED. */
+enum { CT_STAGE_RADON = 0, CT_STAGE_FBP = 1 };
+int ct_hip_check_shape(int stage, int w, int h, int bins, int filter_backend);
 
 /* Timings of the last volume call, milliseconds, measured with HIP events on
  * the context's stream: [0] normalise+Radon, [1] row filter,

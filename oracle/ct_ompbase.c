@@ -37,6 +37,8 @@
 int ct_omp_reconstruct_volume(const float* vol, size_t w, size_t h, size_t d,
                               size_t na, int filter, float* out, int threads,
                               double* phase_ms);
+int ct_omp_radon_forward(const float* slice, size_t w, size_t h, size_t na, size_t bins,
+                         float* sino);
 
 /* RadonTransform.cpp:25-113 with use_parallel=false (the call made from the
  * volume loop, OpenMPBackend.cpp:113). */
@@ -84,6 +86,17 @@ static void radon_forward(const float* slice, size_t w, size_t h, size_t na, siz
             }
         }
     }
+}
+
+/* The volume loop's Radon stage of one slice (min/max as OpenMPBackend.cpp:101-113
+ * take them), exported so the tests can pin it by closed forms. */
+int ct_omp_radon_forward(const float* slice, size_t w, size_t h, size_t na, size_t bins,
+                         float* sino) {
+    if (w == 0 || h == 0 || na == 0 || bins == 0) return -1;
+    float mn = FLT_MAX, mx = -FLT_MAX;
+    for (size_t i = 0; i < w * h; ++i) { if (slice[i] < mn) mn = slice[i]; if (slice[i] > mx) mx = slice[i]; }
+    radon_forward(slice, w, h, na, bins, mn, mx, sino);
+    return 0;
 }
 
 typedef struct { const float* rp; const float* rn; float du, ubp, ubn, dudy; } bdata;

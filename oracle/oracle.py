@@ -269,6 +269,7 @@ def ompbase():
         L = ctypes.CDLL(path)
         L.ct_omp_reconstruct_volume.argtypes = [_f32p, _sz, _sz, _sz, _sz, ctypes.c_int, _f32p,
                                                 ctypes.c_int, _f64p]
+        L.ct_omp_radon_forward.argtypes = [_f32p, _sz, _sz, _sz, _sz, _f32p]
         L.isa = tag
         _omp = L
     return _omp
@@ -282,6 +283,16 @@ def omp_reconstruct_volume(vol, na: int, ftype: int, threads: int):
     _check(ompbase().ct_omp_reconstruct_volume(_p32(vol), w, h, d, na, ftype, _p32(out), threads,
                                                _p64(ph)), "omp_reconstruct_volume")
     return out, ph
+
+
+def omp_radon_forward(slice2d, na: int, bins: int) -> np.ndarray:
+    """The OpenMP baseline's Radon stage of one slice (RadonTransform.cpp:25-113):
+    -sin convention, (int) truncation; [na][bins]."""
+    s = _c32(slice2d)
+    h, w = s.shape
+    out = np.empty((na, bins), np.float32)
+    _check(ompbase().ct_omp_radon_forward(_p32(s), w, h, na, bins, _p32(out)), "omp_radon_forward")
+    return out
 
 
 def metrics(orig, recon) -> dict:

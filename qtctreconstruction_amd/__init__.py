@@ -85,6 +85,7 @@ class SinoInfo(ctypes.Structure):
 
 
 CT_BATCH_GRAPH, CT_BATCH_AUTOTUNE = 1, 2
+CT_STAGE_RADON, CT_STAGE_FBP = 0, 1
 SERVE_FIFO, SERVE_SIZE_AGED = 0, 1
 _lib = None
 
@@ -147,6 +148,7 @@ def load_library(path: str = LIB_PATH):
         "ct_hip_noisy_log_dev": (i, [ctx, _vp, sz, ctypes.c_float, ctypes.c_float, ctypes.c_uint64]),
         "ct_hip_filter_sinograms": (i, [ctx, _f32p, i, i, i, i, _f32p]),
         "ct_hip_last_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double)]),
+        "ct_hip_check_shape": (i, [i, i, i, i, i]),
         "ct_hip_last_radon_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int)]),
         "ct_hip_volume_metrics": (i, [ctx, _f32p, _f32p, sz, ctypes.POINTER(Metrics)]),
@@ -183,7 +185,7 @@ def exported_symbols() -> List[str]:
         "ct_hip_server_wait",
         "ct_hip_server_download", "ct_hip_server_release", "ct_hip_server_stats",
         "ct_hip_server_last_error", "ct_hip_radon_dev", "ct_hip_noisy_log_dev",
-        "ct_hip_last_timings",
+        "ct_hip_last_timings", "ct_hip_check_shape",
         "ct_hip_last_radon_timings", "ct_hip_volume_metrics", "ct_hip_volume_metrics_dev",
         "ct_hip_save_volume_npy", "ct_hip_load_volume_npy", "ct_hip_save_point_cloud_npy",
         "ct_hip_load_point_cloud_npy", "ct_hip_export_sinograms", "ct_hip_load_sinograms",
@@ -196,6 +198,13 @@ def _p(a: np.ndarray):
 
 def _f32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def check_shape(stage: int, w: int, h: int, bins: int, filter_backend: int = 0) -> int:
+    """ct_hip_check_shape: CT_OK, CT_ERR_INVALID or CT_ERR_UNSUPPORTED for a
+    stage's shape (host only, no device needed)."""
+    return int(load_library().ct_hip_check_shape(int(stage), int(w), int(h), int(bins),
+                                                 int(filter_backend)))
 
 
 def phantom_params(seed: int, n: int):
@@ -355,6 +364,22 @@ class Context:
         out = np.empty((nz, na, sq), np.float32)
         self._call("ct_hip_filter_sinograms", self.handle, _p(s), nz, na, bins, int(ftype), _p(out))
         return out
+
+    def reconstruct_volume_dev(self, d_vol: DeviceBuffer, w: int, h: int, d: int, na: int,
+                               ftype: int, d_out: DeviceBuffer,
+                               d_sino: Optional[DeviceBuffer] = None):
+        """reconstructVolume over device memory (ct_hip_reconstruct_volume_dev);
+        d_sino (optional, d*na*w floats) receives the sinograms."""
+        self._call("ct_hip_reconstruct_volume_dev", self.handle, d_vol.ptr, w, h, d, na, int(ftype),
+                   d_out.ptr, d_sino.ptr if d_sino is not None else None)
+
+    def reconstruct_from_sinograms_dev(self, d_sinos: DeviceBuffer, nz: int, na: int, bins: int,
+                                       angles, ftype: int, d_out: DeviceBuffer):
+        """reconstructVolumeFromSinograms over device memory: (nz,na,bins) ->
+        (nz,bins,bins), no denormalisation (ct_hip_reconstruct_from_sinograms_dev)."""
+        a = _f32(angles)
+        self._call("ct_hip_reconstruct_from_sinograms_dev", self.handle, d_sinos.ptr, nz, na, bins,
+                   _p(a), int(ftype), d_out.ptr)
 
     def radon_dev(self, d_vol: DeviceBuffer, w: int, h: int, d: int, na: int, bins: int,
                   d_sino: DeviceBuffer) -> np.ndarray:

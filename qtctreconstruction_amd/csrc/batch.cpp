@@ -198,6 +198,7 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
                    int flags) {
     need(jobs != nullptr && nj > 0, "fbpBatch: no jobs");
     need(nj <= kMaxBatch, "fbpBatch: too many jobs in one batch");
+    trim_trig(c);  // before this batch's tables are looked up, never between
     long long work_best = -1;
     int dom = 0;
     for (int i = 0; i < nj; ++i) {

@@ -69,11 +69,35 @@ void release(ct_hip_ctx* c, Buf& b) {
 // then stays resident and is read through the scalar cache.
 float deg2rad(float d) { return d * 3.14159265358979323846f / 180.0f; }
 
+// At most kTrigCacheLists angle lists keep their tables on the device; the
+// least recently used beyond that are freed at the start of the next call
+// (trim_trig), never in the middle of one, so every table a call or a batch
+// enqueues stays valid until the call returns. hipFree waits for the device,
+// so a table still read by work in flight on another stream is freed after
+// that work.
+constexpr size_t kTrigCacheLists = 32;
+
+void trim_trig(ct_hip_ctx* c) {
+    while (c->trig.size() > kTrigCacheLists) {
+        auto victim = c->trig.begin();
+        for (auto it = c->trig.begin(); it != c->trig.end(); ++it)
+            if (it->second.last_use < victim->second.last_use) victim = it;
+        dfree(c, victim->second.cos);
+        dfree(c, victim->second.sin);
+        dfree(c, victim->second.alist);
+        if (c->cur.cos == victim->second.cos) c->cur = ct_hip_ctx::Trig();
+        c->trig.erase(victim);
+    }
+}
+
 ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
     std::vector<uint32_t> key(na);
     std::memcpy(key.data(), angles_deg, sizeof(float) * na);
     auto it = c->trig.find(key);
-    if (it != c->trig.end()) return it->second;
+    if (it != c->trig.end()) {
+        it->second.last_use = ++c->trig_tick;
+        return it->second;
+    }
     std::vector<float> cs(na), sn(na);
     for (int a = 0; a < na; ++a) {
         const float th = deg2rad(angles_deg[a]);
@@ -95,6 +119,7 @@ ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
     t.sin = static_cast<float*>(dmalloc(c, sizeof(float) * na));
     t.alist = static_cast<int*>(dmalloc(c, sizeof(int) * na));
     t.n_row = n_row;
+    t.last_use = ++c->trig_tick;
     hip_check(hipMemcpyAsync(t.cos, cs.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
               "upload cos table");
     hip_check(hipMemcpyAsync(t.sin, sn.data(), sizeof(float) * na, hipMemcpyHostToDevice, c->stream),
@@ -106,7 +131,12 @@ ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
     return c->trig[std::move(key)] = t;
 }
 
-void set_trig(ct_hip_ctx* c, const float* angles_deg, int na) { c->cur = get_trig(c, angles_deg, na); }
+// Sets the current call's table; trimming first keeps at most
+// kTrigCacheLists tables on the device.
+void set_trig(ct_hip_ctx* c, const float* angles_deg, int na) {
+    trim_trig(c);
+    c->cur = get_trig(c, angles_deg, na);
+}
 
 std::vector<float> serial_angles(int na) {
     std::vector<float> a(na);
@@ -165,11 +195,54 @@ void check_fft(const Geom& g) {
                               std::to_string(g.nfft) + " > " + std::to_string(kMaxFFT));
 }
 
+// Shape limits, checked before any device work. Radon: the walk's candidate
+// intervals carry a margin (kDelta in radon.hip) proven against the float
+// error of Serial's s for edges up to kRadonMaxEdge and bins up to
+// kRadonMaxBins; larger shapes are refused rather than risk a lost deposit.
+int shape_status(int stage, int w, int h, int bins, int backend, std::string* why) {
+    auto fail = [&](int code, const std::string& m) {
+        if (why) *why = m;
+        return code;
+    };
+    if (stage == CT_STAGE_RADON) {
+        if (w <= 0 || h <= 0 || bins <= 0) return fail(CT_ERR_INVALID, "Radon: empty shape");
+        if (w > kRadonMaxEdge || h > kRadonMaxEdge || bins > kRadonMaxBins)
+            return fail(CT_ERR_UNSUPPORTED,
+                        "Radon: " + std::to_string(w) + " x " + std::to_string(h) + " px, " +
+                            std::to_string(bins) + " bins is beyond the supported " +
+                            std::to_string(kRadonMaxEdge) + " px edge / " +
+                            std::to_string(kRadonMaxBins) + " bins");
+        return CT_OK;
+    }
+    if (stage == CT_STAGE_FBP) {
+        if (bins <= 0) return fail(CT_ERR_INVALID, "FBP: empty shape");
+        if (backend == CT_FILTER_BACKEND_FFT_F32 && geometry(bins).nfft > kMaxFFT)
+            return fail(CT_ERR_UNSUPPORTED, "detector width " + std::to_string(bins) +
+                                                " needs an FFT longer than " +
+                                                std::to_string(kMaxFFT));
+        return CT_OK;
+    }
+    return fail(CT_ERR_INVALID, "unknown stage");
+}
+
+void check_radon(int w, int h, int bins) {
+    std::string why;
+    const int rc = shape_status(CT_STAGE_RADON, w, h, bins, 0, &why);
+    if (rc != CT_OK) throw StatusError(rc, why);
+}
+
+void check_fbp(const ct_hip_ctx* c, int bins) {
+    std::string why;
+    const int rc = shape_status(CT_STAGE_FBP, 0, 0, bins, c->filter_backend, &why);
+    if (rc != CT_OK) throw StatusError(rc, why);
+}
+
 size_t nzb_of(int nz) { return (size_t)((nz + kZB - 1) / kZB); }
 
 // normalise + Radon of nz slices (w x h) into d_sino [nz][na][bins]; d_mm gets {min,max} per slice
 void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na, int bins,
                  float* d_sino, float* d_mm) {
+    check_radon(w, h, bins);
     unsigned* keys = ensure<unsigned>(c, c->keys, sizeof(unsigned) * 2 * (size_t)nz);
     const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
     float4* row = ensure<float4>(c, c->row, rzb * radon_zbytes(w, h));
@@ -251,7 +324,7 @@ void stage_filter(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, 
 void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int N, int filter,
                const float* d_mm, float* d_out) {
     const Geom g = geometry(bins);
-    if (c->filter_backend == CT_FILTER_BACKEND_FFT_F32) check_fft(g);
+    check_fbp(c, bins);
     const int cfg = c->bp_config_fixed ? c->bp_config : autotune(c, N, na, nullptr);
     float4* filt = ensure<float4>(c, c->filt, sizeof(float4) * nzb_of(nz) * (size_t)na * g.sq);
     hip_check(hipEventRecord(c->ev[1], c->stream), "event");
@@ -262,6 +335,22 @@ void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int
                                  cfg),
               "backprojection");
     hip_check(hipEventRecord(c->ev[3], c->stream), "event");
+}
+
+// Timings of a volume call that ran the Radon stage only (h != w): the
+// filter and backprojection entries are 0.
+void read_radon_only_timings(ct_hip_ctx* c) {
+    hip_check(hipEventSynchronize(c->ev[1]), "event sync");
+    float t01 = 0, r0 = 0, r1 = 0;
+    hip_check(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]), "elapsed");
+    hip_check(hipEventElapsedTime(&r0, c->evr[0], c->evr[1]), "elapsed");
+    hip_check(hipEventElapsedTime(&r1, c->evr[1], c->evr[2]), "elapsed");
+    c->radon_ms[0] = r0;
+    c->radon_ms[1] = r1;
+    c->timings[0] = t01;
+    c->timings[1] = 0;
+    c->timings[2] = 0;
+    c->timings[3] = t01;
 }
 
 void read_timings(ct_hip_ctx* c, bool with_radon) {
@@ -540,6 +629,7 @@ int ct_hip_radon_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, int
     return guarded(c, [&] {
         need(d_vol && d_sino, "radon: null pointer");
         need(w > 0 && h > 0 && d > 0 && na > 0 && bins > 0, "radon: empty input");
+        check_radon(w, h, bins);
         const std::vector<float> ang = serial_angles(na);
         set_trig(c, ang.data(), na);
         float* d_mm = ensure<float>(c, c->mm, sizeof(float) * 2 * (size_t)d);
@@ -557,6 +647,7 @@ int ct_hip_compute_sinogram(ct_hip_ctx* c, const float* slice, int w, int h, int
     return guarded(c, [&] {
         need(slice && sino_out, "computeSinogram: null pointer");
         need(w > 0 && h > 0 && na > 0 && bins > 0, "computeSinogram: empty input");
+        check_radon(w, h, bins);
         const size_t pix = (size_t)w * h;
         float* d_in = ensure<float>(c, c->vol, sizeof(float) * pix);
         float* d_sino = ensure<float>(c, c->sino, sizeof(float) * (size_t)na * bins);
@@ -584,6 +675,7 @@ int ct_hip_reconstruct_slice(ct_hip_ctx* c, const float* sino, int na, int bins,
     return guarded(c, [&] {
         need(sino && angles_deg && out, "reconstructSlice: null pointer");
         need(na > 0 && bins > 0 && N > 0, "reconstructSlice: empty input");
+        check_fbp(c, bins);
         const size_t n_in = (size_t)na * bins;
         float* d_sino = ensure<float>(c, c->sino, sizeof(float) * n_in);
         float* d_out = ensure<float>(c, c->out, sizeof(float) * (size_t)N * N);
@@ -599,10 +691,17 @@ int ct_hip_reconstruct_slice(ct_hip_ctx* c, const float* sino, int na, int bins,
     });
 }
 
+// The shapes a volume call will run, checked before any allocation or launch.
+static void check_volume(ct_hip_ctx* c, int w, int h, int d, int na) {
+    need(w > 0 && h > 0 && d > 0 && na > 0, "reconstructVolume: empty input");
+    check_radon(w, h, w);  // bins = w, SerialBackend.cpp:287-288
+    if (h == w) check_fbp(c, w);
+}
+
 static void volume_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, int na, int filter,
                        float* d_out, float* d_sino_out) {
     need(d_vol && d_out, "reconstructVolume: null pointer");
-    need(w > 0 && h > 0 && d > 0 && na > 0, "reconstructVolume: empty input");
+    check_volume(c, w, h, d, na);
     const int bins = w;  // SerialBackend.cpp:287-288
     const std::vector<float> ang = serial_angles(na);
     set_trig(c, ang.data(), na);
@@ -615,8 +714,11 @@ static void volume_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, i
         stage_fbp(c, d_sino, d, na, bins, w, filter, d_mm, d_out);
         read_timings(c, true);
     } else {
-        // Buffer3D::setSlice drops slices whose shape differs (CTTypes.h:112-116)
+        // Buffer3D::setSlice drops slices whose shape differs (CTTypes.h:112-116);
+        // Serial still spends the sinogram time, which is what is recorded
+        hip_check(hipEventRecord(c->ev[1], c->stream), "event");
         hip_check(hipMemsetAsync(d_out, 0, sizeof(float) * (size_t)w * h * d, c->stream), "zero");
+        read_radon_only_timings(c);
         hip_check(hipStreamSynchronize(c->stream), "sync");
     }
 }
@@ -633,7 +735,7 @@ int ct_hip_reconstruct_volume(ct_hip_ctx* c, const float* vol, int w, int h, int
                               int filter, float* out, float* sino_out) {
     return guarded(c, [&] {
         need(vol && out, "reconstructVolume: null pointer");
-        need(w > 0 && h > 0 && d > 0 && na > 0, "reconstructVolume: empty input");
+        check_volume(c, w, h, d, na);
         const size_t n = (size_t)w * h * d;
         float* d_vol = ensure<float>(c, c->vol, sizeof(float) * n);
         float* d_out = ensure<float>(c, c->out, sizeof(float) * n);
@@ -654,6 +756,7 @@ static void from_sinos_dev(ct_hip_ctx* c, const float* d_sinos, int nz, int na, 
                            const float* angles_deg, int filter, float* d_out) {
     need(d_sinos && angles_deg && d_out, "reconstructVolumeFromSinograms: null pointer");
     need(nz > 0 && na > 0 && bins > 0, "reconstructVolumeFromSinograms: empty input");
+    check_fbp(c, bins);
     set_trig(c, angles_deg, na);
     stage_fbp(c, d_sinos, nz, na, bins, bins, filter, nullptr, d_out);
     read_timings(c, false);
@@ -670,6 +773,7 @@ int ct_hip_reconstruct_from_sinograms(ct_hip_ctx* c, const float* sinos, int nz,
     return guarded(c, [&] {
         need(sinos && out, "reconstructVolumeFromSinograms: null pointer");
         need(nz > 0 && na > 0 && bins > 0, "reconstructVolumeFromSinograms: empty input");
+        check_fbp(c, bins);
         const size_t nin = (size_t)nz * na * bins, nout = (size_t)nz * bins * bins;
         float* d_in = ensure<float>(c, c->sino, sizeof(float) * nin);
         float* d_out = ensure<float>(c, c->out, sizeof(float) * nout);
@@ -819,6 +923,7 @@ int ct_hip_export_sinograms(ct_hip_ctx* c, const float* vol, int w, int h, int d
     return guarded(c, [&] {
         need(vol && npy_path, "exportSinograms: null pointer");
         need(w > 0 && h > 0 && d > 0 && na > 0, "exportSinograms: empty input");
+        check_radon(w, h, w);
         const size_t pix = (size_t)w * h * d, ns = (size_t)d * na * w;
         float* d_vol = ensure<float>(c, c->vol, sizeof(float) * pix);
         float* d_sino = ensure<float>(c, c->sino, sizeof(float) * ns);
@@ -866,6 +971,7 @@ int ct_hip_filter_sinograms(ct_hip_ctx* c, const float* sinos, int nz, int na, i
     return guarded(c, [&] {
         need(sinos && out, "filterSinograms: null pointer");
         need(nz > 0 && na > 0 && bins > 0, "filterSinograms: empty input");
+        check_fbp(c, bins);
         const Geom g = geometry(bins);
         const size_t nin = (size_t)nz * na * bins;
         const size_t nf = nzb_of(nz) * (size_t)na * g.sq;
@@ -891,6 +997,10 @@ int ct_hip_last_timings(const ct_hip_ctx* c, double* ms4) {
     if (!c || !ms4) return CT_ERR_INVALID;
     for (int i = 0; i < 4; ++i) ms4[i] = c->timings[i];
     return CT_OK;
+}
+
+int ct_hip_check_shape(int stage, int w, int h, int bins, int filter_backend) {
+    return shape_status(stage, w, h, bins, filter_backend, nullptr);
 }
 
 int ct_hip_last_radon_timings(const ct_hip_ctx* c, double* ms2, int* launches) {

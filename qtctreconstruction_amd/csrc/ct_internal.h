@@ -75,6 +75,15 @@ inline int xcd_group(int nslot, int nunit, double target = 300.0) {
 
 // Largest FFT length of the LDS row filter (bins <= 1448).
 constexpr int kMaxFFT = 4096;
+// Largest image edge and detector count of the Radon walk: the candidate
+// margin kDelta (radon.hip) covers the float error of Serial's s, below
+// ~6e-4 of a bin for |s| < 4096, and the row stepping error, ~1e-3 px for
+// rows of up to 2048 px, stays below the walk's 0.01 px margin. The walk's
+// 32-bit byte offsets into one z-block stay below 2^32 well within these.
+constexpr int kRadonMaxEdge = 2048;
+constexpr int kRadonMaxBins = 4096;
+static_assert((unsigned long long)(kRadonMaxEdge + kPadX) * kRadonMaxEdge * 32 < (1ull << 32),
+              "the Radon walk's 32-bit offsets must cover a whole z-block");
 
 // ---- phantom -------------------------------------------------------------
 // Per-ellipsoid parameters prepared on the host: centre, world->body map

@@ -78,9 +78,11 @@ struct ct_hip_ctx {
         // (row walk over the image), then the rest (over the transpose)
         int* alist = nullptr;
         int n_row = 0;
+        uint64_t last_use = 0;  // trig_tick of the last call that used the table
     };
     std::map<std::vector<uint32_t>, Trig> trig;  // keyed by the angle list's bit pattern
     Trig cur;                                     // table of the current plain call
+    uint64_t trig_tick = 0;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     double timings[4] = {0, 0, 0, 0};
     // Radon stage breakdown: events before the walk over the image, between
@@ -115,6 +117,8 @@ void need(bool ok, const char* msg);
 int norm_filter(int f);
 Geom geometry(int bins);
 void check_fft(const Geom& g);
+void check_fbp(const ct_hip_ctx* c, int bins);
+void trim_trig(ct_hip_ctx* c);
 ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na);
 float* get_filter(ct_hip_ctx* c, int nfft, int type);
 float2* get_twiddles(ct_hip_ctx* c, int nfft);

@@ -28,19 +28,29 @@
 
 namespace cthip {
 
-__device__ __forceinline__ unsigned f2key(float f) {
+// Order-preserving map of floats to unsigned keys (for -0 < +0 too; the
+// extremes then differ from Serial's only in the sign of a zero, which every
+// later use treats alike). Serial's scan (SerialBackend.cpp:69-76) starts
+// from FLT_MAX / lowest and compares with < and >, so a NaN never becomes an
+// extreme: kmin/kmax send NaN to the keys of the other end, which lose every
+// comparison, and an all-NaN (or empty) slice keeps FLT_MAX / -FLT_MAX.
+__host__ __device__ __forceinline__ unsigned f2key(float f) {
     const unsigned u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 __device__ __forceinline__ float key2f(unsigned k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
+__device__ __forceinline__ unsigned kmin_of(float f) { return f != f ? 0xffffffffu : f2key(f); }
+__device__ __forceinline__ unsigned kmax_of(float f) { return f != f ? 0u : f2key(f); }
+constexpr unsigned kKeyFltMax = 0xff7fffffu;   // f2key(FLT_MAX)
+constexpr unsigned kKeyFltLowest = 0x00800000u;  // f2key(-FLT_MAX)
 
 __global__ void minmax_init_kernel(unsigned* keys, int nz) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nz) {
-        keys[2 * i] = 0xffffffffu;
-        keys[2 * i + 1] = 0u;
+        keys[2 * i] = kKeyFltMax;
+        keys[2 * i + 1] = kKeyFltLowest;
     }
 }
 
@@ -66,23 +76,21 @@ __global__ __launch_bounds__(256) void minmax_kernel(const float* __restrict__ v
             for (int u = 0; u < 4; ++u) q[u] = p4[i + u * stride];
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const unsigned a = f2key(q[u].x), b = f2key(q[u].y), c = f2key(q[u].z),
-                               d = f2key(q[u].w);
-                kmin = min(kmin, min(min(a, b), min(c, d)));
-                kmax = max(kmax, max(max(a, b), max(c, d)));
+                kmin = min(kmin, min(min(kmin_of(q[u].x), kmin_of(q[u].y)),
+                                     min(kmin_of(q[u].z), kmin_of(q[u].w))));
+                kmax = max(kmax, max(max(kmax_of(q[u].x), kmax_of(q[u].y)),
+                                     max(kmax_of(q[u].z), kmax_of(q[u].w))));
             }
         }
         for (; i < n4; i += stride) {
             const float4 q = p4[i];
-            const unsigned a = f2key(q.x), b = f2key(q.y), c = f2key(q.z), d = f2key(q.w);
-            kmin = min(kmin, min(min(a, b), min(c, d)));
-            kmax = max(kmax, max(max(a, b), max(c, d)));
+            kmin = min(kmin, min(min(kmin_of(q.x), kmin_of(q.y)), min(kmin_of(q.z), kmin_of(q.w))));
+            kmax = max(kmax, max(max(kmax_of(q.x), kmax_of(q.y)), max(kmax_of(q.z), kmax_of(q.w))));
         }
     } else {
         for (size_t i = first; i < slice_elems; i += stride) {
-            const unsigned k = f2key(p[i]);
-            kmin = min(kmin, k);
-            kmax = max(kmax, k);
+            kmin = min(kmin, kmin_of(p[i]));
+            kmax = max(kmax, kmax_of(p[i]));
         }
     }
     for (int off = 32; off > 0; off >>= 1) {

@@ -25,12 +25,14 @@ public:
     virtual void reconstructVolume(
         const Volume& input_volume, Volume& out_reconstruction, const ReconstructionParams& params,
         std::function<void(int slice_idx, const Buffer2D& recon_slice)> onSliceDone) = 0;
-    // The reference gives this a default per-slice body (BackendFactory.cpp:17-60);
-    // HipBackend overrides it with one batched device call.
+    // Default body as the reference's (BackendFactory.cpp:17-60): one
+    // reconstructSlice per z onto bins x bins, each slice's min/max taken from
+    // its sinogram; defined in BackendFactory.cpp. HipBackend overrides it with
+    // one batched device call.
     virtual void reconstructVolumeFromSinograms(
         const Volume& sinograms, const std::vector<float>& angles_deg, Volume& out_reconstruction,
         const ReconstructionParams& params,
-        std::function<void(int slice_idx, const Buffer2D& recon_slice)> onSliceDone) = 0;
+        std::function<void(int slice_idx, const Buffer2D& recon_slice)> onSliceDone);
     virtual PointCloud extractPointCloud(const Volume& vol, float threshold) = 0;
     virtual double lastSinogramTimeMs() const = 0;
     virtual double lastReconstructionTimeMs() const { return 0.0; }

@@ -70,6 +70,13 @@ int main(int argc, char** argv) {
         Volume rec2;
         backend->reconstructVolumeFromSinograms(sinos, s.angles_deg, rec2, params, nullptr);
         dump(out + "/fromsino.f32", rec2.data.data(), rec2.data.size());
+        // the interface's default body (one reconstructSlice per z), which a
+        // backend without its own override would run
+        Volume rec_default;
+        int default_calls = 0;
+        backend->IReconstructionBackend::reconstructVolumeFromSinograms(
+            sinos, s.angles_deg, rec_default, params, [&](int, const Buffer2D&) { ++default_calls; });
+        dump(out + "/fromsino_default.f32", rec_default.data.data(), rec_default.data.size());
 
         vol.x_coords.resize(w);
         for (size_t x = 0; x < w; ++x) vol.x_coords[x] = 0.5f * (float)x;
@@ -111,6 +118,7 @@ int main(int argc, char** argv) {
              << vol_rt << "\n"
              << pc_rt << "\n";
         meta.precision(17);
+        meta << default_calls << "\n";
         meta << m.mse << "\n" << m.mae << "\n" << m.psnr << "\n" << m.max_abs << "\n" << m.vmin
              << "\n" << m.vmax << "\n" << m.count << "\n";
     } catch (const std::exception& ex) {

@@ -62,3 +62,21 @@ def test_product_does_not_import_oracle():
                 txt = open(os.path.join(dirpath, f), errors="replace").read()
                 assert "oracle" not in re.sub(r"#.*|//.*", "", txt).lower() or f == "__init__.py" \
                     and "import oracle" not in txt, f
+
+
+def test_shape_limits_refused_without_a_device():
+    """ct_hip_check_shape is the check every entry point makes before any
+    device work: Radon edges beyond 2048 px or more than 4096 bins, and FFT
+    filters longer than 4096 (bins > 1448), are refused; sizes <= 0 are invalid."""
+    import qtctreconstruction_amd as ct
+    R, F = ct.CT_STAGE_RADON, ct.CT_STAGE_FBP
+    assert ct.check_shape(R, 2048, 2048, 4096) == ct.CT_OK
+    assert ct.check_shape(R, 2049, 16, 16) == ct.CT_ERR_UNSUPPORTED
+    assert ct.check_shape(R, 16, 2049, 16) == ct.CT_ERR_UNSUPPORTED
+    assert ct.check_shape(R, 16, 16, 4097) == ct.CT_ERR_UNSUPPORTED
+    assert ct.check_shape(R, 0, 16, 16) == ct.CT_ERR_INVALID
+    assert ct.check_shape(F, 0, 0, 1448) == ct.CT_OK          # nfft 4096
+    assert ct.check_shape(F, 0, 0, 1449) == ct.CT_ERR_UNSUPPORTED
+    assert ct.check_shape(F, 0, 0, 1449, int(ct.FilterBackend.GEMM_SPLIT_BF16)) == ct.CT_OK
+    assert ct.check_shape(F, 0, 0, 0) == ct.CT_ERR_INVALID
+    assert ct.check_shape(7, 1, 1, 1) == ct.CT_ERR_INVALID

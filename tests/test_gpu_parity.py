@@ -214,8 +214,72 @@ def test_volume_sinograms_every_slice(backend, O, n, d, na):
 
 def test_volume_non_square_slices_stay_zero(backend, O):
     vol = O.phantom(3, 8, 40, 24, 4)  # h != w: Buffer3D::setSlice drops every slice
-    out = backend.reconstructVolume(vol, ct.ReconstructionParams(ct.FilterType.Ramp, 20))
+    out, sino = backend.reconstructVolume(vol, ct.ReconstructionParams(ct.FilterType.Ramp, 20),
+                                          want_sinograms=True)
     assert np.all(out == 0.0)
+    # the sinograms are still Serial's (bins = w), and the timings are this
+    # call's: the Radon stage, and 0 for the filter and the backprojection
+    for z in range(4):
+        ref, _ = O.compute_sinogram(vol[z], 20, 40)
+        _sino_parity(O, sino[z], ref)
+    t = backend.ctx.timings()
+    assert t[0] > 0.0 and t[1] == 0.0 and t[2] == 0.0 and t[3] == t[0]
+    assert backend.lastReconstructionTimeMs() == 0.0
+
+
+def test_nan_pixel_is_not_an_extreme(backend, O):
+    """Serial's min/max scan skips NaN (SerialBackend.cpp:69-76): one NaN pixel
+    leaves original_min/max_hu at the finite extremes, on the slice entry and
+    the volume path."""
+    rng = np.random.default_rng(5)
+    s = _slice(rng, 40, 40)
+    s[7, 9] = np.nan
+    got = backend.computeSinogram(s, 12, 40)
+    _, mm = O.compute_sinogram(s, 12, 40)
+    assert _bits_equal([got.original_min_hu, got.original_max_hu], mm)
+    vol = np.stack([_slice(rng, 40, 40) for _ in range(9)])
+    vol[4, 20, 30] = np.nan
+    vol[8, :, :] = np.nan  # all NaN: Serial keeps FLT_MAX / lowest
+    d_vol = backend.ctx.alloc(vol.nbytes).upload(vol)
+    d_sino = backend.ctx.alloc(4 * 9 * 12 * 40)
+    mmv = backend.ctx.radon_dev(d_vol, 40, 40, 9, 12, 40, d_sino)
+    for z in range(9):
+        _, ref = O.compute_sinogram(vol[z], 12, 40)
+        assert _bits_equal(mmv[z], ref), z
+    d_vol.free()
+    d_sino.free()
+
+
+def test_trig_cache_is_bounded(backend, O):
+    """Every distinct angle list gets a device table; beyond 32 lists the least
+    recently used are freed, so device memory stays flat over many lists."""
+    rng = np.random.default_rng(8)
+    sino = rng.random((8, 16)).astype(np.float32)
+    params = ct.ReconstructionParams(ct.FilterType.Ramp, 8)
+
+    def call(k):
+        ang = np.sort(rng.uniform(0.0, 180.0, 8)).astype(np.float32)
+        return backend.reconstructSlice(ct.Sinogram(sino, ang), 16, params), ang
+
+    for k in range(40):
+        call(k)
+    used = backend.ctx.bytes_in_use()
+    for k in range(200):
+        out, ang = call(k)
+    assert backend.ctx.bytes_in_use() == used
+    # an evicted list is rebuilt correctly when it comes back
+    assert O.rel_l2(out, O.reconstruct_slice(sino, ang, 16, O.RAMP)) <= TOL
+
+
+def test_radon_shape_limit_is_an_error(backend):
+    """Edges beyond the proven 2048 px are refused before any device work."""
+    with pytest.raises(ct.BackendError) as ei:
+        backend.computeSinogram(np.ones((1, 2049), np.float32), 4, 16)
+    assert ei.value.code == ct.CT_ERR_UNSUPPORTED
+    with pytest.raises(ct.BackendError) as ei:
+        backend.reconstructVolume(np.ones((1, 2, 2049), np.float32),
+                                  ct.ReconstructionParams(ct.FilterType.Ramp, 4))
+    assert ei.value.code == ct.CT_ERR_UNSUPPORTED
 
 
 def test_from_sinograms(backend, O):
@@ -326,8 +390,17 @@ def test_cpp_hip_backend_through_the_interface(O, tmp_path):
     assert _bits_equal(ff, fs)
     assert meta[6] == "1" and meta[7] == "1"  # volume and point-cloud NPY round trips
     np.testing.assert_array_equal(np.load(tmp_path / "volume.npy"), v)
+    # the interface's default reconstructVolumeFromSinograms body: one
+    # reconstructSlice per z (the FFT then pairs each row with zeros instead of
+    # the next slice's row, so the bits may differ by rounding)
+    assert int(meta[8]) == d
+    fd = np.fromfile(tmp_path / "fromsino_default.f32", np.float32).reshape(d, w, w)
+    assert O.rel_l2(fd, fs) <= 1e-6
+    for z in range(d):
+        s, _ = O.compute_sinogram(vol[z], na, w)
+        assert O.rel_l2(fd[z], O.reconstruct_slice(s, O.serial_angles(na), w, filt)) <= TOL
     m = O.metrics(vol, v)
-    got = [float(x) for x in meta[8:15]]
+    got = [float(x) for x in meta[9:16]]
     for k, g in zip(("mse", "mae", "psnr"), got[:3]):
         assert g == pytest.approx(m[k], rel=1e-9), k
     assert np.float32(got[3]) == m["max_abs"] and np.float32(got[4]) == m["vmin"]

@@ -326,3 +326,109 @@ def test_phantom_additive(O):
         one = (O.Ellipsoid * 1)(e[k])
         acc += O.phantom_from(one, 1, 20, 20, 20)
     np.testing.assert_allclose(vol_all, acc, atol=1e-6)
+
+
+def test_sinogram_minmax_skips_nan(O):
+    # Serial's scan compares with < and > from FLT_MAX / lowest
+    # (SerialBackend.cpp:69-76): a NaN pixel never becomes an extreme
+    rng = np.random.default_rng(17)
+    s = _random_slice(rng, 24, 24)
+    s[3, 4] = np.nan
+    _, mm = O.compute_sinogram(s, 6, 24)
+    assert mm[0] == np.nanmin(s) and mm[1] == np.nanmax(s)
+    _, mm = O.compute_sinogram(np.full((4, 4), np.nan, np.float32), 3, 4)
+    assert mm[0] == np.finfo(np.float32).max and mm[1] == -np.finfo(np.float32).max
+
+
+# ----------------------------------------------------------- OpenMP baseline
+# The timed CPU baseline restates OpenMPBackend::reconstructVolume
+# (OpenMPBackend.cpp:61-190, RadonTransform.cpp:25-113,
+# OptimizedBackprojectionCPU.cpp:22-174). Its conventions differ from
+# Serial's, and each check below fails if the restatement took Serial's
+# instead: -sin in both directions (no y-flip), (int) truncation of the bin,
+# the (sq-1)/2 detector centre of the backprojection.
+
+@pytest.mark.parametrize("w", [32, 33])
+def test_omp_radon_angle0_is_column_sums(O, w):
+    rng = np.random.default_rng(21)
+    s = np.abs(_random_slice(rng, w, w))
+    sino = O.omp_radon_forward(s, 4, w)
+    v = s.astype(np.float64) / s.max()  # min 0 (zeros present), max -> 1
+    col = v.sum(axis=0)
+    ref = np.zeros(w)
+    if w % 2 == 0:  # u = x + 1/2: half to bin x, half to x + 1; the last bin takes all
+        ref[:-1] += 0.5 * col[:-1]
+        ref[1:] += 0.5 * col[:-1]
+        ref[-1] += col[-1]
+    else:  # u = x exactly: the whole pixel to bin x
+        ref[:] = col
+    np.testing.assert_allclose(sino[0], ref, rtol=1e-5, atol=1e-5)
+
+
+def test_omp_radon_angle90_reverses_rows(O):
+    # -sin: at 90 degrees row y lands on bin ~ (h - 1 - y) + 1/2, the reverse
+    # of Serial's +sin order
+    rng = np.random.default_rng(22)
+    n = 32
+    s = np.abs(_random_slice(rng, n, n))
+    sino = O.omp_radon_forward(s, 4, n)  # a = 2 -> 90 degrees
+    v = s.astype(np.float64) / s.max()
+    row = v.sum(axis=1)
+    ref = np.zeros(n)
+    ref[n - 1] += row[0]  # u = n - 1/2: the last bin takes all
+    for y in range(1, n):
+        ref[n - 1 - y] += 0.5 * row[y]
+        ref[n - y] += 0.5 * row[y]
+    np.testing.assert_allclose(sino[2], ref, rtol=1e-4, atol=1e-4)
+    serial, _ = O.compute_sinogram(s, 4, n)
+    assert O.rel_l2(sino[2], serial[2]) > 0.1  # the two conventions really differ
+
+
+def test_omp_radon_truncates_negative_bins(O):
+    # pixel (0, 15) of a 16 x 16 slice at 10 degrees: u = 8 - 7.5 (cos + sin)
+    # = -0.69. (int) truncation sends it to bins 0 and 1 with weights 1 - u > 1
+    # and u < 0 (mass kept); floor would drop it
+    s = np.zeros((16, 16), np.float32)
+    s[15, 0] = 1.0
+    sino = O.omp_radon_forward(s, 18, 16)  # a = 1 -> 10 degrees
+    row = sino[1]
+    assert row[0] > 1.0 and row[1] < 0.0 and np.all(row[2:] == 0.0)
+    assert abs(row.sum() - 1.0) < 1e-6
+
+
+def _omp_disk_volume(n, d, cx, cy, r):
+    y, x = np.mgrid[0:n, 0:n]
+    sl = np.where((x - cx) ** 2 + (y - cy) ** 2 <= r * r, 1.0, 0.0).astype(np.float32)
+    return np.repeat(sl[None], d, axis=0)
+
+
+def test_omp_volume_disk_unflipped(O):
+    # no y-flip: a disk at (x0, y0) reconstructs at (x0, y0) with amplitude ~1
+    # (min 0, max 1, so denormalisation is the identity); sq = 91 is odd, so
+    # the (sq - 1)/2 centre meets the forward's bins/2 exactly
+    n, d = 64, 4
+    vol = _omp_disk_volume(n, d, 20.0, 40.0, 5.0)
+    out, ph = O.omp_reconstruct_volume(vol, 180, O.RAMP, 2)
+    assert ph[0] > 0 and ph[1] > 0 and ph[2] >= ph[0] + ph[1]
+    y, x = np.mgrid[0:n, 0:n]
+    for z in range(d):
+        rec = out[z]
+        wgt = np.where(rec > 0.5 * rec.max(), rec, 0.0)
+        cx, cy = (wgt * x).sum() / wgt.sum(), (wgt * y).sum() / wgt.sum()
+        assert abs(cx - 20.0) < 0.3 and abs(cy - 40.0) < 0.3, (z, cx, cy)
+        assert abs(rec[40, 20] - 1.0) < 0.1, rec[40, 20]
+
+
+def test_omp_volume_linearity(O):
+    # with min 0 the per-slice normalisation and denormalisation cancel, so the
+    # baseline is linear in the volume (to float rounding)
+    rng = np.random.default_rng(23)
+    n, d = 40, 3
+    v1 = np.abs(rng.standard_normal((d, n, n))).astype(np.float32)
+    v2 = np.abs(rng.standard_normal((d, n, n))).astype(np.float32)
+    v1[:, 0, 0] = 0.0
+    v2[:, 0, 0] = 0.0
+    a, _ = O.omp_reconstruct_volume(v1, 36, O.HANN, 2)
+    b, _ = O.omp_reconstruct_volume(v2, 36, O.HANN, 2)
+    c, _ = O.omp_reconstruct_volume(v1 + v2, 36, O.HANN, 2)
+    assert O.rel_l2(c, a.astype(np.float64) + b) < 1e-5
