@@ -149,6 +149,7 @@ def load_library(path: str = LIB_PATH):
         "ct_hip_filter_sinograms": (i, [ctx, _f32p, i, i, i, i, _f32p]),
         "ct_hip_last_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double)]),
         "ct_hip_check_shape": (i, [i, i, i, i, i]),
+        "ct_hip_last_kernel_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double), i]),
         "ct_hip_last_radon_timings": (i, [ctx, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int)]),
         "ct_hip_volume_metrics": (i, [ctx, _f32p, _f32p, sz, ctypes.POINTER(Metrics)]),
@@ -185,7 +186,7 @@ def exported_symbols() -> List[str]:
         "ct_hip_server_wait",
         "ct_hip_server_download", "ct_hip_server_release", "ct_hip_server_stats",
         "ct_hip_server_last_error", "ct_hip_radon_dev", "ct_hip_noisy_log_dev",
-        "ct_hip_last_timings", "ct_hip_check_shape",
+        "ct_hip_last_timings", "ct_hip_check_shape", "ct_hip_last_kernel_timings",
         "ct_hip_last_radon_timings", "ct_hip_volume_metrics", "ct_hip_volume_metrics_dev",
         "ct_hip_save_volume_npy", "ct_hip_load_volume_npy", "ct_hip_save_point_cloud_npy",
         "ct_hip_load_point_cloud_npy", "ct_hip_export_sinograms", "ct_hip_load_sinograms",
@@ -307,6 +308,14 @@ class Context:
         t = (ctypes.c_double * 4)()
         self._call("ct_hip_last_timings", self.handle, t)
         return np.array(t[:], np.float64)
+
+    def kernel_timings(self) -> dict:
+        """Per-kernel milliseconds of the last call (summed over its slabs)."""
+        t = (ctypes.c_double * 6)()
+        self._call("ct_hip_last_kernel_timings", self.handle, t, 6)
+        keys = ["minmax", "normalise", "walk_image", "walk_transpose", "row_filter",
+                "backprojection"]
+        return dict(zip(keys, [float(x) for x in t]))
 
     def radon_timings(self):
         """(ms of the row walk over the image, ms of the row walk over the
@@ -463,7 +472,12 @@ class Server:
         return t.value, lat.value
 
     def download(self, ticket: int, shape) -> np.ndarray:
-        out = np.empty(shape, np.float32)
+        return self.download_into(ticket, np.empty(shape, np.float32))
+
+    def download_into(self, ticket: int, out: np.ndarray) -> np.ndarray:
+        """A finished job's output into a caller's float32 C-contiguous array."""
+        if out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError("download_into: float32 C-contiguous array required")
         rc = self.lib.ct_hip_server_download(self.handle, ticket, _p(out), out.nbytes)
         if rc != CT_OK:
             self._err(rc, "download")

@@ -250,23 +250,25 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     const size_t nside = std::max(radon_side_floats(w, bins, nz, na),
                                   radon_side_floats(h, bins, nz, na));
     float* side = nside ? ensure<float>(c, c->side, sizeof(float) * nside) : nullptr;
+    hip_check(hipEventRecord(c->evr[0], c->stream), "event");
     hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
+    hip_check(hipEventRecord(c->evr[1], c->stream), "event");
     hip_check(launch_normalise(d_vol, w, h, nz, d_mm, row, col, c->stream), "normalise");
     // The two walks write disjoint angles of the sinogram. They run one after
     // the other: side by side on two streams they evict each other's cache
     // lines (measured 24% slower at 1024^3, no gain at 512^3).
-    hip_check(hipEventRecord(c->evr[0], c->stream), "event");
+    hip_check(hipEventRecord(c->evr[2], c->stream), "event");
     const int n_row = c->cur.n_row;
     hip_check(launch_radon_rows(row, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
                                 n_row, d_sino, side, c->stream),
               "radon rows");
-    hip_check(hipEventRecord(c->evr[1], c->stream), "event");
+    hip_check(hipEventRecord(c->evr[3], c->stream), "event");
     hip_check(launch_radon_rows(col, h, w, nz, na, bins, c->cur.sin, c->cur.cos,
                                 c->cur.alist + n_row, na - n_row, d_sino, side, c->stream),
               "radon rows of the transpose");
-    hip_check(hipEventRecord(c->evr[2], c->stream), "event");
+    hip_check(hipEventRecord(c->evr[4], c->stream), "event");
     c->radon_timed = true;
 }
 
@@ -320,12 +322,19 @@ void stage_filter(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, 
               "toeplitz gemm");
 }
 
+// Backprojection tile configuration for (N, na): the fixed one, or the
+// autotuner's (which times candidates on the context stream the first time a
+// class is seen, so callers ask before their first timing event).
+int bp_config_for(ct_hip_ctx* c, int N, int na) {
+    return c->bp_config_fixed ? c->bp_config : autotune(c, N, na, nullptr);
+}
+
 // filter + backprojection of nz sinograms [nz][na][bins] onto N x N
 void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int N, int filter,
                const float* d_mm, float* d_out) {
     const Geom g = geometry(bins);
     check_fbp(c, bins);
-    const int cfg = c->bp_config_fixed ? c->bp_config : autotune(c, N, na, nullptr);
+    const int cfg = bp_config_for(c, N, na);
     float4* filt = ensure<float4>(c, c->filt, sizeof(float4) * nzb_of(nz) * (size_t)na * g.sq);
     hip_check(hipEventRecord(c->ev[1], c->stream), "event");
     stage_filter(c, d_sino, nz, na, bins, filter, filt);
@@ -337,39 +346,47 @@ void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int
     hip_check(hipEventRecord(c->ev[3], c->stream), "event");
 }
 
-// Timings of a volume call that ran the Radon stage only (h != w): the
-// filter and backprojection entries are 0.
-void read_radon_only_timings(ct_hip_ctx* c) {
-    hip_check(hipEventSynchronize(c->ev[1]), "event sync");
-    float t01 = 0, r0 = 0, r1 = 0;
-    hip_check(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]), "elapsed");
-    hip_check(hipEventElapsedTime(&r0, c->evr[0], c->evr[1]), "elapsed");
-    hip_check(hipEventElapsedTime(&r1, c->evr[1], c->evr[2]), "elapsed");
-    c->radon_ms[0] = r0;
-    c->radon_ms[1] = r1;
-    c->timings[0] = t01;
-    c->timings[1] = 0;
-    c->timings[2] = 0;
-    c->timings[3] = t01;
+void zero_timings(ct_hip_ctx* c) {
+    for (double& t : c->timings) t = 0;
+    for (double& t : c->radon_ms) t = 0;
+    c->radon_slabs = 0;
 }
 
-void read_timings(ct_hip_ctx* c, bool with_radon) {
-    hip_check(hipEventSynchronize(c->ev[3]), "event sync");
+// Adds the stage times recorded in the current event set: the Radon stage
+// (ev[0] -> ev[1], its walks in evr) when with_radon, the filter (ev[1] ->
+// ev[2]) and the backprojection (ev[2] -> ev[3]) when with_fbp.
+void add_timings(ct_hip_ctx* c, bool with_radon, bool with_fbp) {
+    hip_check(hipEventSynchronize(with_fbp ? c->ev[3] : c->ev[1]), "event sync");
     float t01 = 0, t12 = 0, t23 = 0;
     if (with_radon) {
+        ++c->radon_slabs;
         hip_check(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]), "elapsed");
-        float r0 = 0, r1 = 0;
-        hip_check(hipEventElapsedTime(&r0, c->evr[0], c->evr[1]), "elapsed");
-        hip_check(hipEventElapsedTime(&r1, c->evr[1], c->evr[2]), "elapsed");
-        c->radon_ms[0] = r0;
-        c->radon_ms[1] = r1;
+        for (int k = 0; k < 4; ++k) {
+            float r = 0;
+            hip_check(hipEventElapsedTime(&r, c->evr[k], c->evr[k + 1]), "elapsed");
+            c->radon_ms[k] += r;
+        }
     }
-    hip_check(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]), "elapsed");
-    hip_check(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]), "elapsed");
-    c->timings[0] = t01;
-    c->timings[1] = t12;
-    c->timings[2] = t23;
-    c->timings[3] = (double)t01 + t12 + t23;
+    if (with_fbp) {
+        hip_check(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]), "elapsed");
+        hip_check(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]), "elapsed");
+    }
+    c->timings[0] += t01;
+    c->timings[1] += t12;
+    c->timings[2] += t23;
+    c->timings[3] += (double)t01 + t12 + t23;
+}
+
+// Whole-buffer host transfers through the pinned stager, ordered after the
+// work already enqueued on the context's stream; both return when done.
+void to_device(ct_hip_ctx* c, void* d_dst, const void* h_src, size_t bytes) {
+    hip_check(hipEventRecord(c->pev[1][0], c->stream), "event");
+    stager(c).upload(d_dst, h_src, bytes, c->pev[1][0]);
+}
+
+void to_host(ct_hip_ctx* c, void* h_dst, const void* d_src, size_t bytes) {
+    hip_check(hipEventRecord(c->pev[1][0], c->stream), "event");
+    stager(c).download(h_dst, d_src, bytes, c->pev[1][0]);
 }
 
 void bind_device(ct_hip_ctx* c) { hip_check(hipSetDevice(c->device), "hipSetDevice"); }
@@ -474,17 +491,18 @@ int ct_hip_create(int device, size_t hbm_budget_bytes, ct_hip_ctx** out) {
         delete c;
         return CT_ERR_HIP;
     }
-    for (auto& e : c->evr) {
-        if (hipEventCreate(&e) != hipSuccess) {
-            ct_hip_destroy(c);
-            return CT_ERR_HIP;
-        }
-    }
-    for (auto& e : c->ev) {
-        if (hipEventCreate(&e) != hipSuccess) {
-            ct_hip_destroy(c);
-            return CT_ERR_HIP;
-        }
+    bool ok = hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->d2h, hipStreamNonBlocking) == hipSuccess;
+    for (auto& set : c->evs)
+        for (auto& e : set) ok = ok && hipEventCreate(&e) == hipSuccess;
+    for (auto& set : c->evrs)
+        for (auto& e : set) ok = ok && hipEventCreate(&e) == hipSuccess;
+    for (auto& set : c->pev)
+        for (auto& e : set)
+            ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        ct_hip_destroy(c);
+        return CT_ERR_HIP;
     }
     *out = c;
     return CT_OK;
@@ -498,12 +516,25 @@ void ct_hip_destroy(ct_hip_ctx* c) {
         cthost::batch_release(c);
     } catch (...) {
     }
+    if (c->h2d) (void)hipStreamSynchronize(c->h2d);
+    if (c->d2h) (void)hipStreamSynchronize(c->d2h);
+    try {
+        cthost::release_host_paths(c);
+    } catch (...) {
+    }
     for (auto& kv : c->sizes) (void)hipFree(kv.first);
     c->sizes.clear();
-    for (auto& e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    for (auto& e : c->evr)
-        if (e) (void)hipEventDestroy(e);
+    for (auto& set : c->evs)
+        for (auto& e : set)
+            if (e) (void)hipEventDestroy(e);
+    for (auto& set : c->evrs)
+        for (auto& e : set)
+            if (e) (void)hipEventDestroy(e);
+    for (auto& set : c->pev)
+        for (auto& e : set)
+            if (e) (void)hipEventDestroy(e);
+    if (c->h2d) (void)hipStreamDestroy(c->h2d);
+    if (c->d2h) (void)hipStreamDestroy(c->d2h);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -535,6 +566,7 @@ int ct_hip_clear_workspace(ct_hip_ctx* c) {
         c->toeplitz.clear();
         release(c, c->rmax);
         batch_release(c);
+        release_host_paths(c);
         for (auto& kv : c->twiddles) dfree(c, kv.second);
         c->filters.clear();
         c->twiddles.clear();
@@ -556,19 +588,21 @@ int ct_hip_free(ct_hip_ctx* c, void* p) {
     });
 }
 
+// Both go through the context's pinned stager, ordered after the work
+// already enqueued on the context's stream.
 int ct_hip_upload(ct_hip_ctx* c, void* dst, const void* src, size_t bytes) {
     return guarded(c, [&] {
         need(dst && src, "upload: null pointer");
-        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "upload");
-        hip_check(hipStreamSynchronize(c->stream), "upload sync");
+        hip_check(hipEventRecord(c->pev[1][0], c->stream), "event");
+        stager(c).upload(dst, src, bytes, c->pev[1][0]);
     });
 }
 
 int ct_hip_download(ct_hip_ctx* c, void* dst, const void* src, size_t bytes) {
     return guarded(c, [&] {
         need(dst && src, "download: null pointer");
-        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "download");
-        hip_check(hipStreamSynchronize(c->stream), "download sync");
+        hip_check(hipEventRecord(c->pev[1][0], c->stream), "event");
+        stager(c).download(dst, src, bytes, c->pev[1][0]);
     });
 }
 
@@ -611,7 +645,7 @@ int ct_hip_phantom(ct_hip_ctx* c, const ct_ellipsoid* e, int n, int nx, int ny, 
         const size_t bytes = sizeof(float) * (size_t)nx * ny * nz;
         float* d = ensure<float>(c, c->vol, bytes);
         phantom_to_device(c, e, n, nx, ny, nz, d);
-        hip_check(hipMemcpy(host_out, d, bytes, hipMemcpyDeviceToHost), "download phantom");
+        to_host(c, host_out, d, bytes);
     });
 }
 
@@ -698,6 +732,24 @@ static void check_volume(ct_hip_ctx* c, int w, int h, int d, int na) {
     if (h == w) check_fbp(c, w);
 }
 
+// Slices per slab of the device path: the whole volume when its workspace
+// fits the context's budget, else the most slices (a multiple of 8, so the
+// Radon z-blocks and the filter's row pairs are those of the whole-volume
+// run and the result is the same bit for bit) whose workspace does.
+static int volume_dev_slab(ct_hip_ctx* c, int w, int h, int d, int na, bool sino_ws) {
+    if (!c->budget) return d;
+    const size_t per = volume_ws_per_slice(w, h, na, sino_ws);
+    size_t held = 0;  // workspaces the call reuses count as available
+    for (const Buf* b : {&c->row, &c->col, &c->filt, &c->sino}) held += b->cap;
+    const size_t avail = c->budget > c->in_use - held ? c->budget - (c->in_use - held) : 0;
+    const size_t fit = avail / std::max<size_t>(per, 1) * 15 / 16;  // margin for the small buffers
+    if (fit >= (size_t)d) return d;
+    if (fit < (size_t)std::min(d, kRZ))
+        throw StatusError(CT_ERR_BUDGET, "reconstructVolume: the HBM budget holds no slab of " +
+                                             std::to_string(std::min(d, kRZ)) + " slices");
+    return (int)(fit / kRZ * kRZ);
+}
+
 static void volume_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, int na, int filter,
                        float* d_out, float* d_sino_out) {
     need(d_vol && d_out, "reconstructVolume: null pointer");
@@ -705,22 +757,30 @@ static void volume_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, i
     const int bins = w;  // SerialBackend.cpp:287-288
     const std::vector<float> ang = serial_angles(na);
     set_trig(c, ang.data(), na);
-    float* d_sino = d_sino_out ? d_sino_out
-                               : ensure<float>(c, c->sino, sizeof(float) * (size_t)d * na * bins);
+    if (h == w) (void)bp_config_for(c, w, na);
+    zero_timings(c);
+    const int S = volume_dev_slab(c, w, h, d, na, d_sino_out == nullptr);
+    const size_t wh = (size_t)w * h, sl_sino = (size_t)na * bins;
     float* d_mm = ensure<float>(c, c->mm, sizeof(float) * 2 * (size_t)d);
-    hip_check(hipEventRecord(c->ev[0], c->stream), "event");
-    stage_radon(c, d_vol, w, h, d, na, bins, d_sino, d_mm);
-    if (h == w) {
-        stage_fbp(c, d_sino, d, na, bins, w, filter, d_mm, d_out);
-        read_timings(c, true);
-    } else {
-        // Buffer3D::setSlice drops slices whose shape differs (CTTypes.h:112-116);
-        // Serial still spends the sinogram time, which is what is recorded
-        hip_check(hipEventRecord(c->ev[1], c->stream), "event");
-        hip_check(hipMemsetAsync(d_out, 0, sizeof(float) * (size_t)w * h * d, c->stream), "zero");
-        read_radon_only_timings(c);
-        hip_check(hipStreamSynchronize(c->stream), "sync");
+    float* ws_sino = d_sino_out ? nullptr : ensure<float>(c, c->sino, sizeof(float) * S * sl_sino);
+    for (int z0 = 0; z0 < d; z0 += S) {
+        const int nz = std::min(S, d - z0);
+        float* sino = d_sino_out ? d_sino_out + (size_t)z0 * sl_sino : ws_sino;
+        hip_check(hipEventRecord(c->ev[0], c->stream), "event");
+        stage_radon(c, d_vol + z0 * wh, w, h, nz, na, bins, sino, d_mm + 2 * (size_t)z0);
+        if (h == w) {
+            stage_fbp(c, sino, nz, na, bins, w, filter, d_mm + 2 * (size_t)z0, d_out + z0 * wh);
+            add_timings(c, true, true);
+        } else {
+            // Buffer3D::setSlice drops slices whose shape differs (CTTypes.h:112-116);
+            // Serial still spends the sinogram time, which is what is recorded
+            hip_check(hipEventRecord(c->ev[1], c->stream), "event");
+            hip_check(hipMemsetAsync(d_out + z0 * wh, 0, sizeof(float) * nz * wh, c->stream),
+                      "zero");
+            add_timings(c, true, false);
+        }
     }
+    hip_check(hipStreamSynchronize(c->stream), "sync");
 }
 
 int ct_hip_reconstruct_volume_dev(ct_hip_ctx* c, const float* d_vol, int w, int h, int d, int na,
@@ -736,19 +796,7 @@ int ct_hip_reconstruct_volume(ct_hip_ctx* c, const float* vol, int w, int h, int
     return guarded(c, [&] {
         need(vol && out, "reconstructVolume: null pointer");
         check_volume(c, w, h, d, na);
-        const size_t n = (size_t)w * h * d;
-        float* d_vol = ensure<float>(c, c->vol, sizeof(float) * n);
-        float* d_out = ensure<float>(c, c->out, sizeof(float) * n);
-        hip_check(hipMemcpyAsync(d_vol, vol, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
-                  "upload volume");
-        volume_dev(c, d_vol, w, h, d, na, filter, d_out, nullptr);
-        hip_check(hipMemcpyAsync(out, d_out, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream),
-                  "download volume");
-        if (sino_out)
-            hip_check(hipMemcpyAsync(sino_out, c->sino.p, sizeof(float) * (size_t)d * na * w,
-                                     hipMemcpyDeviceToHost, c->stream),
-                      "download sinograms");
-        hip_check(hipStreamSynchronize(c->stream), "sync");
+        volume_host(c, vol, w, h, d, na, filter, out, sino_out);
     });
 }
 
@@ -758,8 +806,11 @@ static void from_sinos_dev(ct_hip_ctx* c, const float* d_sinos, int nz, int na, 
     need(nz > 0 && na > 0 && bins > 0, "reconstructVolumeFromSinograms: empty input");
     check_fbp(c, bins);
     set_trig(c, angles_deg, na);
+    (void)bp_config_for(c, bins, na);
+    zero_timings(c);
+    hip_check(hipEventRecord(c->ev[1], c->stream), "event");
     stage_fbp(c, d_sinos, nz, na, bins, bins, filter, nullptr, d_out);
-    read_timings(c, false);
+    add_timings(c, false, true);
 }
 
 int ct_hip_reconstruct_from_sinograms_dev(ct_hip_ctx* c, const float* d_sinos, int nz, int na,
@@ -771,18 +822,10 @@ int ct_hip_reconstruct_from_sinograms_dev(ct_hip_ctx* c, const float* d_sinos, i
 int ct_hip_reconstruct_from_sinograms(ct_hip_ctx* c, const float* sinos, int nz, int na, int bins,
                                       const float* angles_deg, int filter, float* out) {
     return guarded(c, [&] {
-        need(sinos && out, "reconstructVolumeFromSinograms: null pointer");
+        need(sinos && angles_deg && out, "reconstructVolumeFromSinograms: null pointer");
         need(nz > 0 && na > 0 && bins > 0, "reconstructVolumeFromSinograms: empty input");
         check_fbp(c, bins);
-        const size_t nin = (size_t)nz * na * bins, nout = (size_t)nz * bins * bins;
-        float* d_in = ensure<float>(c, c->sino, sizeof(float) * nin);
-        float* d_out = ensure<float>(c, c->out, sizeof(float) * nout);
-        hip_check(hipMemcpyAsync(d_in, sinos, sizeof(float) * nin, hipMemcpyHostToDevice, c->stream),
-                  "upload sinograms");
-        from_sinos_dev(c, d_in, nz, na, bins, angles_deg, filter, d_out);
-        hip_check(hipMemcpyAsync(out, d_out, sizeof(float) * nout, hipMemcpyDeviceToHost, c->stream),
-                  "download volume");
-        hip_check(hipStreamSynchronize(c->stream), "sync");
+        from_sinos_host(c, sinos, nz, na, bins, angles_deg, filter, out);
     });
 }
 
@@ -853,8 +896,7 @@ int ct_hip_extract_point_cloud(ct_hip_ctx* c, const float* vol, int w, int h, in
         need(w > 0 && h > 0 && d > 0, "extractPointCloud: empty input");
         const size_t n = (size_t)w * h * d;
         float* d_vol = ensure<float>(c, c->vol, sizeof(float) * n);
-        hip_check(hipMemcpyAsync(d_vol, vol, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
-                  "upload volume");
+        to_device(c, d_vol, vol, sizeof(float) * n);
         point_cloud_dev(c, d_vol, w, h, d, xc, yc, zc, threshold, out, cap, count);
     });
 }
@@ -906,10 +948,8 @@ int ct_hip_volume_metrics(ct_hip_ctx* c, const float* orig, const float* recon, 
         need(orig && recon, "volumeMetrics: null pointer");
         float* d_o = ensure<float>(c, c->vol, sizeof(float) * n);
         float* d_r = ensure<float>(c, c->out, sizeof(float) * n);
-        hip_check(hipMemcpyAsync(d_o, orig, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
-                  "upload original");
-        hip_check(hipMemcpyAsync(d_r, recon, sizeof(float) * n, hipMemcpyHostToDevice, c->stream),
-                  "upload reconstruction");
+        to_device(c, d_o, orig, sizeof(float) * n);
+        to_device(c, d_r, recon, sizeof(float) * n);
         metrics_dev(c, d_o, d_r, n, out);
     });
 }
@@ -930,13 +970,10 @@ int ct_hip_export_sinograms(ct_hip_ctx* c, const float* vol, int w, int h, int d
         float* d_mm = ensure<float>(c, c->mm, sizeof(float) * 2 * (size_t)d);
         const std::vector<float> ang = serial_angles(na);
         set_trig(c, ang.data(), na);
-        hip_check(hipMemcpyAsync(d_vol, vol, sizeof(float) * pix, hipMemcpyHostToDevice, c->stream),
-                  "upload volume");
+        to_device(c, d_vol, vol, sizeof(float) * pix);
         stage_radon(c, d_vol, w, h, d, na, w, d_sino, d_mm);
         std::vector<float> sino(ns), mm(2 * (size_t)d);
-        hip_check(hipMemcpyAsync(sino.data(), d_sino, sizeof(float) * ns, hipMemcpyDeviceToHost,
-                                 c->stream),
-                  "download sinograms");
+        to_host(c, sino.data(), d_sino, sizeof(float) * ns);
         hip_check(hipMemcpyAsync(mm.data(), d_mm, sizeof(float) * mm.size(), hipMemcpyDeviceToHost,
                                  c->stream),
                   "download min/max");
@@ -977,13 +1014,10 @@ int ct_hip_filter_sinograms(ct_hip_ctx* c, const float* sinos, int nz, int na, i
         const size_t nf = nzb_of(nz) * (size_t)na * g.sq;
         float* d_in = ensure<float>(c, c->sino, sizeof(float) * nin);
         float4* filt = ensure<float4>(c, c->filt, sizeof(float4) * nf);
-        hip_check(hipMemcpyAsync(d_in, sinos, sizeof(float) * nin, hipMemcpyHostToDevice, c->stream),
-                  "upload sinograms");
+        to_device(c, d_in, sinos, sizeof(float) * nin);
         stage_filter(c, d_in, nz, na, bins, filter, filt);
         std::vector<float> tmp(4 * nf);
-        hip_check(hipMemcpyAsync(tmp.data(), filt, sizeof(float4) * nf, hipMemcpyDeviceToHost,
-                                 c->stream),
-                  "download filtered");
+        to_host(c, tmp.data(), filt, sizeof(float4) * nf);
         hip_check(hipStreamSynchronize(c->stream), "sync");
         for (int z = 0; z < nz; ++z)
             for (int a = 0; a < na; ++a)
@@ -1006,9 +1040,17 @@ int ct_hip_check_shape(int stage, int w, int h, int bins, int filter_backend) {
 int ct_hip_last_radon_timings(const ct_hip_ctx* c, double* ms2, int* launches) {
     if (!c || !ms2) return CT_ERR_INVALID;
     if (!c->radon_timed) return CT_ERR_INVALID;
-    ms2[0] = c->radon_ms[0];
-    ms2[1] = c->radon_ms[1];
-    if (launches) *launches = 1;
+    ms2[0] = c->radon_ms[2];
+    ms2[1] = c->radon_ms[3];
+    if (launches) *launches = c->radon_slabs;
+    return CT_OK;
+}
+
+int ct_hip_last_kernel_timings(const ct_hip_ctx* c, double* ms, int n) {
+    if (!c || !ms || n < 0) return CT_ERR_INVALID;
+    const double v[6] = {c->radon_ms[0], c->radon_ms[1], c->radon_ms[2], c->radon_ms[3],
+                         c->timings[1], c->timings[2]};
+    for (int i = 0; i < n && i < 6; ++i) ms[i] = v[i];
     return CT_OK;
 }
 

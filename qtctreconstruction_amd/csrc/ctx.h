@@ -6,9 +6,13 @@
 #include "../../include/ct_hip.h"
 #include "ct_internal.h"
 
+#include <atomic>
+#include <condition_variable>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -32,6 +36,62 @@ struct Geom {
 };
 
 struct BatchGraph;  // batch.cpp
+
+// Parallel memcpy between pageable caller memory and pinned staging: the
+// calling thread and `workers` pool threads each take chunks of one copy at a
+// time (hostpath.cpp). Below 4 MB the caller copies alone.
+class CopyPool {
+public:
+    explicit CopyPool(int workers);
+    ~CopyPool();
+    CopyPool(const CopyPool&) = delete;
+    CopyPool& operator=(const CopyPool&) = delete;
+    void copy(void* dst, const void* src, size_t bytes);
+
+private:
+    void work();
+    void loop();
+    std::vector<std::thread> th_;
+    size_t nworkers_ = 0;
+    std::mutex job_mu_;  // one copy at a time
+    std::mutex mu_;
+    std::condition_variable cv_, cv_done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0, chunk_ = 0, nchunks_ = 0;
+    std::atomic<size_t> next_{0};
+    size_t finished_ = 0;  // workers done with the current copy
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// Host <-> device copies through two pinned chunks on a stream of its own:
+// the host copies chunk i + 1 into (or out of) pinned memory while the DMA
+// engine moves chunk i. Synchronous; `after` (optional) is an event the DMA
+// waits for first (the producer of the data, or the last reader of the
+// destination). One transfer at a time per stager (its own lock).
+struct Stager {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    char* pin[2] = {nullptr, nullptr};
+    size_t cap = 0;  // bytes of each pinned chunk
+    CopyPool* pool = nullptr;
+    bool ready = false;
+    void init(CopyPool* p);
+    void upload(void* d_dst, const void* h_src, size_t bytes, hipEvent_t after);
+    void download(void* h_dst, const void* d_src, size_t bytes, hipEvent_t after);
+    ~Stager();
+};
+
+// Pinned slab buffers of the pipelined host-buffer paths (hostpath.cpp).
+struct HostSlabs {
+    char* in[2] = {nullptr, nullptr};
+    char* out[2] = {nullptr, nullptr};
+    char* sino[2] = {nullptr, nullptr};
+    size_t in_cap = 0, out_cap = 0, sino_cap = 0;
+    ~HostSlabs();
+};
 
 // Largest number of volumes in one batch.
 constexpr int kMaxBatch = 256;
@@ -83,12 +143,20 @@ struct ct_hip_ctx {
     std::map<std::vector<uint32_t>, Trig> trig;  // keyed by the angle list's bit pattern
     Trig cur;                                     // table of the current plain call
     uint64_t trig_tick = 0;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // Stage events: before the Radon stage, before the filter, before the
+    // backprojection, after it; and for the Radon stage's breakdown, before
+    // the min/max pass, before the normalisation, before the walk over the
+    // image, between the two walks, after the walk over the transpose. Two sets, so a pipelined slab records into one while the
+    // previous slab's are still to be read; ev / evr point at the current set.
+    hipEvent_t evs[2][4] = {};
+    hipEvent_t evrs[2][5] = {};
+    hipEvent_t* ev = evs[0];
+    hipEvent_t* evr = evrs[0];
     double timings[4] = {0, 0, 0, 0};
-    // Radon stage breakdown: events before the walk over the image, between
-    // the two walks and after the walk over the transpose
-    hipEvent_t evr[3] = {nullptr, nullptr, nullptr};
-    double radon_ms[2] = {0, 0};
+    // Radon stage breakdown summed over the call's slabs: min/max, normalise,
+    // walk over the image, walk over the transpose; and the slab count
+    double radon_ms[4] = {0, 0, 0, 0};
+    int radon_slabs = 0;
     bool radon_timed = false;
     int filter_backend = 0;  // CT_FILTER_BACKEND_*
     int bp_config = 0;       // backprojection tile configuration (kBpConfigs index)
@@ -105,6 +173,13 @@ struct ct_hip_ctx {
     std::map<std::pair<int, int>, int> tuned;  // (N, na) -> backprojection config
     bool tune_cache_loaded = false;            // CT_HIP_TUNE_CACHE read into `tuned`
     cthost::BatchGraph* batch = nullptr;
+    // host-buffer paths: copy streams, memcpy threads, pinned staging
+    hipStream_t h2d = nullptr, d2h = nullptr;
+    hipEvent_t pev[3][2] = {};  // per slab parity: H2D done, compute done, D2H done
+    std::unique_ptr<cthost::CopyPool> copier;
+    cthost::Stager stager;
+    cthost::HostSlabs slabs;
+    Buf pin_vol[2], pout_vol[2], psino[2];  // device slab buffers of the pipeline
 };
 
 namespace cthost {
@@ -118,7 +193,32 @@ int norm_filter(int f);
 Geom geometry(int bins);
 void check_fft(const Geom& g);
 void check_fbp(const ct_hip_ctx* c, int bins);
+void check_radon(int w, int h, int bins);
 void trim_trig(ct_hip_ctx* c);
+void set_trig(ct_hip_ctx* c, const float* angles_deg, int na);
+std::vector<float> serial_angles(int na);
+size_t nzb_of(int nz);
+void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na, int bins,
+                 float* d_sino, float* d_mm);
+void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int N, int filter,
+               const float* d_mm, float* d_out);
+int bp_config_for(ct_hip_ctx* c, int N, int na);
+// Adds the stage times of the events in the current set to c->timings: the
+// Radon stage when with_radon, the filter and backprojection when with_fbp.
+void add_timings(ct_hip_ctx* c, bool with_radon, bool with_fbp);
+void zero_timings(ct_hip_ctx* c);
+// The context's memcpy pool and stager, created on first use.
+CopyPool* copier(ct_hip_ctx* c);
+Stager& stager(ct_hip_ctx* c);
+// Pipelined host-buffer paths (hostpath.cpp).
+void volume_host(ct_hip_ctx* c, const float* vol, int w, int h, int d, int na, int filter,
+                 float* out, float* sino_out);
+void from_sinos_host(ct_hip_ctx* c, const float* sinos, int nz, int na, int bins,
+                     const float* angles_deg, int filter, float* out);
+void release_host_paths(ct_hip_ctx* c);
+// Device bytes of the volume path's workspace per slice: the normalised
+// copies, the filtered sinogram, and the sinogram itself when sino_ws.
+size_t volume_ws_per_slice(int w, int h, int na, bool sino_ws);
 ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na);
 float* get_filter(ct_hip_ctx* c, int nfft, int type);
 float2* get_twiddles(ct_hip_ctx* c, int nfft);

@@ -31,6 +31,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <memory>
 #include <thread>
 
 using namespace cthip;
@@ -139,6 +140,10 @@ struct ct_hip_server {
     bool stop = false;
     std::thread worker;
     std::string err;
+    // downloads of finished outputs: pinned staging on a stream of their own,
+    // so a client's download neither takes the context's lock nor waits
+    // behind the worker's launches
+    std::unique_ptr<cthost::Stager> dl;
     // statistics
     uint64_t batches = 0, completed = 0, max_inflight_jobs = 0, overlapped_launches = 0;
     double busy_since = -1, first_submit = -1, last_done = 0;
@@ -335,6 +340,15 @@ int ct_hip_server_create(int device, size_t pool_bytes, int max_batch, int slots
     s->pool.init(pool_bytes);
     s->max_batch = max_batch;
     s->slots.resize(slots);
+    s->dl.reset(new cthost::Stager());
+    rc = guarded(c, [&] { s->dl->init(copier(c)); });
+    if (rc != CT_OK) {
+        s->dl.reset();
+        ct_hip_free(c, p);
+        delete s;
+        ct_hip_destroy(c);
+        return rc;
+    }
     s->worker = std::thread(worker_loop, s);
     *out = s;
     return CT_OK;
@@ -358,6 +372,7 @@ void ct_hip_server_destroy(ct_hip_server* s) {
         }
     }
     for (auto& kv : s->jobs) delete kv.second;
+    s->dl.reset();  // before the context, whose copy threads it uses
     ct_hip_destroy(s->ctx);
     delete s;
 }
@@ -451,7 +466,16 @@ int ct_hip_server_download(ct_hip_server* s, uint64_t ticket, float* host_out, s
         nb = it->second->out_bytes;
     }
     if (bytes < nb) return CT_ERR_INVALID;
-    return ct_hip_download(s->ctx, host_out, src, nb);
+    // the job's batch has completed (state 2), so its output is final
+    try {
+        (void)hipSetDevice(s->ctx->device);
+        s->dl->download(host_out, src, nb, nullptr);
+    } catch (const StatusError& e) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = e.what();
+        return e.code;
+    }
+    return CT_OK;
 }
 
 // stats[0] batches, [1] completed jobs, [2] max jobs in flight,

@@ -321,6 +321,31 @@ def test_hbm_budget_refusal():
     assert ei.value.code == ct.CT_ERR_BUDGET
 
 
+def test_budget_forces_ragged_slabs_bit_identical(O):
+    """Under an HBM budget too small for the whole volume's workspace, the
+    device path and the host-buffer path both run z-slabs (multiples of 8
+    slices, the last one ragged) and give the unbudgeted result bit for bit."""
+    n, d, na = 256, 45, 180
+    vol = O.phantom(77, 12, n, n, d)
+    params = ct.ReconstructionParams(ct.FilterType.Hann, na)
+    ref, ref_sino = ct.HipBackend(0).reconstructVolume(vol, params, want_sinograms=True)
+    nbytes = vol.nbytes
+    # host path: ~2 MB of device memory per slice -> slabs of 16
+    hb = ct.HipBackend(0, hbm_budget_bytes=40 << 20)
+    out, sino = hb.reconstructVolume(vol, params, want_sinograms=True)
+    assert _bits_equal(out, ref) and _bits_equal(sino, ref_sino)
+    assert hb.ctx.bytes_in_use() <= 40 << 20
+    # device path: input and output resident, ~1 MB of workspace per slice
+    ctx = ct.Context(0, hbm_budget_bytes=2 * nbytes + (20 << 20))
+    d_vol = ctx.alloc(nbytes).upload(vol)
+    d_out = ctx.alloc(nbytes)
+    ctx.reconstruct_volume_dev(d_vol, n, n, d, na, int(params.filter), d_out)
+    assert _bits_equal(d_out.download(vol.shape), ref)
+    assert ctx.bytes_in_use() <= 2 * nbytes + (20 << 20)
+    ctx.close()
+    hb.ctx.close()
+
+
 def test_unsupported_width_is_an_error(backend):
     sino = np.ones((4, 3000), np.float32)
     with pytest.raises(ct.BackendError) as ei:
