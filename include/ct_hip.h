@@ -139,6 +139,24 @@ int ct_hip_reconstruct_from_sinograms_dev(ct_hip_ctx* ctx, const float* d_sinos,
                                           int bins, const float* angles_deg, int filter,
                                           float* d_out);
 
+/* Allocation-free form of ct_hip_reconstruct_from_sinograms_dev: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same arguments and, bit for bit, the same output; the
+ * call allocates nothing. Its scratch (angle table, filter taps, twiddles,
+ * filtered rows in slabs of 4-slice z-blocks) is one 64 MiB array per GPU,
+ * held from library load to process end. Limits: the FFT filter backend
+ * only, at most 8192 angles, and one z-block of filtered rows (na x
+ * ceil(sqrt2*bins) x 16 bytes) must fit in 64 MiB less 104 KiB; beyond them
+ * CT_ERR_UNSUPPORTED. The backprojection tiles are the tuned ones when the
+ * (bins, na) class has been tuned on this context, else the context's
+ * default (the same bits either way). Calls on one context are serialised;
+ * calls on two contexts of one GPU that overlap race on the shared array.
+ * Waits for completion. */
+int ct_hip_reconstruct_from_sinograms_dev_noalloc(ct_hip_ctx* ctx, const float* d_sinos, int nz,
+                                                  int na, int bins, const float* angles_deg,
+                                                  int filter, float* d_out);
+
 /* Replaces IReconstructionBackend::extractPointCloud (IReconstructionBackend.h:44;
  * SerialBackend.cpp:325-346): every voxel with value > threshold, in z,y,x
  * order, as 4 floats (x, y, z, hu). xc/yc/zc are the axis coordinates (host

@@ -122,6 +122,7 @@ def load_library(path: str = LIB_PATH):
         "ct_hip_reconstruct_volume_dev": (i, [ctx, _vp, i, i, i, i, i, _vp, _vp]),
         "ct_hip_reconstruct_from_sinograms": (i, [ctx, _f32p, i, i, i, _f32p, i, _f32p]),
         "ct_hip_reconstruct_from_sinograms_dev": (i, [ctx, _vp, i, i, i, _f32p, i, _vp]),
+        "ct_hip_reconstruct_from_sinograms_dev_noalloc": (i, [ctx, _vp, i, i, i, _f32p, i, _vp]),
         "ct_hip_extract_point_cloud": (i, [ctx, _f32p, i, i, i, _f32p, _f32p, _f32p, ctypes.c_float,
                                            _f32p, sz, ctypes.POINTER(sz)]),
         "ct_hip_extract_point_cloud_dev": (i, [ctx, _vp, i, i, i, _f32p, _f32p, _f32p,
@@ -178,6 +179,7 @@ def exported_symbols() -> List[str]:
         "ct_hip_phantom", "ct_hip_phantom_dev", "ct_hip_compute_sinogram",
         "ct_hip_reconstruct_slice", "ct_hip_reconstruct_volume", "ct_hip_reconstruct_volume_dev",
         "ct_hip_reconstruct_from_sinograms", "ct_hip_reconstruct_from_sinograms_dev",
+        "ct_hip_reconstruct_from_sinograms_dev_noalloc",
         "ct_hip_extract_point_cloud", "ct_hip_extract_point_cloud_dev",
         "ct_hip_design_filter", "ct_hip_set_filter_backend", "ct_hip_filter_sinograms",
         "ct_hip_fbp_batch_dev", "ct_hip_bp_config_count", "ct_hip_bp_config", "ct_hip_autotune",

@@ -28,7 +28,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract
 
 SOURCES = ["phantom.hip", "radon.hip", "filter.hip", "backproject.hip", "pointcloud.hip",
            "filter_gemm.hip", "filter_design.cpp", "metrics.hip", "fileio.cpp", "capi.cpp",
-           "batch.cpp", "serve.cpp", "hostpath.cpp"]
+           "batch.cpp", "serve.cpp", "hostpath.cpp", "noalloc.hip"]
 
 
 def _compile(src: str) -> str:

@@ -491,6 +491,7 @@ int ct_hip_create(int device, size_t hbm_budget_bytes, ct_hip_ctx** out) {
         delete c;
         return CT_ERR_HIP;
     }
+    cthost::noalloc_init(device);
     bool ok = hipStreamCreateWithFlags(&c->h2d, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&c->d2h, hipStreamNonBlocking) == hipSuccess;
     for (auto& set : c->evs)

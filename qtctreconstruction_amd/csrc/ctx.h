@@ -203,6 +203,9 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
 void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int N, int filter,
                const float* d_mm, float* d_out);
 int bp_config_for(ct_hip_ctx* c, int N, int na);
+float deg2rad(float d);
+// Takes the allocation-free entry's scratch address on `device` (current).
+void noalloc_init(int device);
 // Adds the stage times of the events in the current set to c->timings: the
 // Radon stage when with_radon, the filter and backprojection when with_fbp.
 void add_timings(ct_hip_ctx* c, bool with_radon, bool with_fbp);
