@@ -510,6 +510,165 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
     }
 }
 
+// The row walk with 16 slices per lane, for rows of up to 516 pixels (512
+// bins a block): a lane sums the same candidate pixels of two z-blocks of the
+// row copy (slices 16 zp .. 16 zp + 15), so one candidate's geometry and
+// weights serve 32 fused multiply-adds instead of 16. A block of 1024 threads
+// walks two angles over the same staged rows (threads 0-511: the pair's first
+// angle, 512-1023: its second), which halves the staging per lane and lets
+// two blocks (8 waves per SIMD) share a CU. Rows are staged by DMA from
+// global memory straight into LDS (global_load_lds_dwordx4: lane l of a wave
+// writes slot base + l; 64 pixels per load, so rows are padded to kR16W = 576
+// slots), one row ahead: row y is summed from one array while row y + 1
+// lands in the other, so the staging latency hides behind a row of sums.
+// Each candidate's 4 b128 LDS reads are issued before its sums. Weights:
+// u = s - i in [0, 1) exactly when floor(s) = i (one unsigned compare of u's
+// bits); A = fl(1 - u), or 1 on the last bin (SerialBackend.cpp:117-118);
+// B = u (the last bin's B goes nowhere and is not written).
+constexpr int kR16W = 576;
+
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void radon_rows16_kernel(
+    const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
+    const float* __restrict__ cos_t, const float* __restrict__ sin_t,
+    const int* __restrict__ alist, float* __restrict__ sino, float* __restrict__ side,
+    int n_ang, int gsz) {
+    constexpr int NT = 512;
+    // staged rows, one row ahead: even rows in tE, odd rows in tO (two
+    // arrays, so the LDS reads of one row need not wait for the DMA into the
+    // other); [slice group (4 slices)][pixel]
+    __shared__ float4 tE[4][kR16W];
+    __shared__ float4 tO[4][kR16W];
+    const int bb = (bins + NT - 1) / NT;
+    const int nzb = (nz + kRZ - 1) / kRZ;
+    const int npair = (nzb + 1) / 2;
+    const int nap = (n_ang + 1) / 2;  // angle pairs
+    const int slots = (bb * nap + gsz - 1) / gsz;
+    int zp, slot;
+    if (!xcd_map(blockIdx.x, gsz, slots, bb * nap, npair, zp, slot)) return;  // whole block
+    const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9);
+    const int lt = threadIdx.x & (NT - 1);
+    const int ai = 2 * (slot / bb) + half;  // this half's angle, in the walk's list
+    const bool angle_live = ai < n_ang;
+    const int a = alist[angle_live ? ai : n_ang - 1];
+    const int jb = slot % bb;
+    const int i = jb * NT + lt;
+    const bool live = angle_live && i < bins;
+    const RayGeom g = ray_geom(i < bins ? i : bins - 1, bins, w, h, cos_t[a], sin_t[a]);
+    const float ka = (i == bins - 1) ? 0.0f : -1.0f;
+    const int wp = w + kPadX;
+    // A row is 4 slice groups x 9 loads of 64 pixels; group grp is half
+    // (grp & 1) of z-block 2 zp + (grp >> 1), the last z-block standing in for
+    // a missing second one (its slices are never written); slots past the
+    // row's pad repeat its last pixel, and no lane reads them.
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const size_t zbytes = radon_zbytes(w, h);
+    // wave v stages group v & 3, pixel chunks v >> 2 and (v >> 2) + 4 (pixels
+    // 0..511), and waves 0-3 the tail chunk 8 (pixels 512..575, clamped) of
+    // group v: every lane offset is fixed for the whole walk
+    const int sgrp = wave & 3, sk = wave >> 2;
+    const char* sbase = reinterpret_cast<const char*>(rowL) +
+                        (size_t)min(2 * zp + (sgrp >> 1), nzb - 1) * zbytes + (sgrp & 1) * 16;
+    const unsigned xo0 = (unsigned)min(sk * 64 + lane, wp - 1) << 5;
+    const unsigned xo1 = (unsigned)min(sk * 64 + 256 + lane, wp - 1) << 5;
+    const unsigned xo2 = (unsigned)min(512 + lane, wp - 1) << 5;
+    const int nload = wave < 4 ? 3 : 2;
+    auto stage = [&](float4 (&t)[4][kR16W], int y) {
+        const char* src = sbase + (size_t)y * (size_t)wp * 32u;
+        // a rolled loop: unrolled, the loads' LDS targets are taken as
+        // possibly read by the row in progress, which then waits for them
+#pragma unroll 1
+        for (int m = 0; m < nload; ++m) {
+            const unsigned xo = m == 0 ? xo0 : (m == 1 ? xo1 : xo2);
+            const int slot = m == 2 ? 512 : sk * 64 + m * 256;
+            __builtin_amdgcn_global_load_lds((const void*)(src + xo),
+                                             (__attribute__((address_space(3))) void*)&t[sgrp][slot],
+                                             16, 0, 0);
+        }
+    };
+    float accA[16], accB[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) accA[k] = accB[k] = 0.0f;
+    auto row = [&](const float4 (&t)[4][kR16W], int y) {
+        const float yf = (float)y;
+        const float sy = (yf - g.cy) * g.sn;  // SerialBackend.cpp:110
+        const float x0f = row_first(g, yf);
+        const int d = (int)x0f;
+        const float xm0 = x0f - g.cx;
+        float4 q[2][4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int gq = 0; gq < 4; ++gq) q[k][gq] = t[gq][d + k];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            // SerialBackend.cpp:110, the same float operations in order
+            const float s = (k == 0 ? xm0 : xm0 + 1.0f) * g.c + sy + g.dc;
+            const float u = s - g.fi;
+            const bool in = __float_as_uint(u) < 0x3f800000u;
+            const float wa = in ? fmaf(u, ka, 1.0f) : 0.0f;
+            const float wb = in ? u : 0.0f;
+            const float p[16] = {q[k][0].x, q[k][0].y, q[k][0].z, q[k][0].w,
+                                 q[k][1].x, q[k][1].y, q[k][1].z, q[k][1].w,
+                                 q[k][2].x, q[k][2].y, q[k][2].z, q[k][2].w,
+                                 q[k][3].x, q[k][3].y, q[k][3].z, q[k][3].w};
+#pragma unroll
+            for (int z = 0; z < 16; ++z) {
+                accA[z] = fmaf(p[z], wa, accA[z]);
+                accB[z] = fmaf(p[z], wb, accB[z]);
+            }
+        }
+    };
+    stage(tE, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    for (int y = 0; y < h; y += 2) {
+        // row y from tE while row y + 1 lands in tO (read last by row y - 1,
+        // before the previous barrier)
+        if (y + 1 < h) stage(tO, y + 1);
+        row(tE, y);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        if (y + 1 < h) {
+            if (y + 2 < h) stage(tE, y + 2);
+            row(tO, y + 1);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+        }
+    }
+    // bin i = A_i + B_(i-1), as in radon_rows_kernel, per half (the staging
+    // arrays are free: every wave passed the last barrier)
+    float4* bx = (half == 0 ? &tE[0][0] : &tO[0][0]);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        bx[4 * lt + q] = make_float4(accB[4 * q], accB[4 * q + 1], accB[4 * q + 2], accB[4 * q + 3]);
+    __syncthreads();
+    if (live) {
+        float pb[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) pb[k] = 0.0f;
+        if (lt > 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 v = bx[4 * (lt - 1) + q];
+                pb[4 * q] = v.x;
+                pb[4 * q + 1] = v.y;
+                pb[4 * q + 2] = v.z;
+                pb[4 * q + 3] = v.w;
+            }
+        }
+        const bool seam = lt == NT - 1 && i + 1 < bins;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int z = zp * 16 + k;
+            if (z < nz) {
+                sino[((size_t)z * na + a) * bins + i] = accA[k] + pb[k];
+                if (seam) side[((size_t)z * na + a) * (bb - 1) + jb] = accB[k];
+            }
+        }
+    }
+}
+
 // Adds each bin block's last B to the next block's first bin (grid: seams x
 // angles of the walk, z).
 __global__ void radon_seam_kernel(float* __restrict__ sino, const float* __restrict__ side,
@@ -573,17 +732,22 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int NT = radon_lanes(w);
     const int bb = (bins + NT - 1) / NT;
-    // XCD group: ~300 blocks per XCD and round for 512-lane and windowed
-    // blocks, proportionally fewer for the wider ones (measured: one XCD per
-    // z-block at 512^3, two at 768^3 and 1024^3)
+    // Rows up to 516 px: the 16-slice walk over angle pairs (units: pairs of
+    // z-blocks; ~150 of its 1024-thread blocks per XCD and round: 40-600
+    // measured alike at 512^3). Wider rows: the 8-slice walk, ~300 blocks per
+    // XCD and round for windowed blocks, proportionally fewer for the wider
+    // whole-row ones (measured: two XCDs per z-block at 768^3 and 1024^3).
     const int gsz = xcd_group(bb * n, nzb, NT > 512 ? 300.0 * 512.0 / NT : 300.0);
     const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
     const dim3 grid((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
-    if (NT == 512)
-        hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 520, 512, true>), grid, dim3(512), 0,
-                           s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
-                           n, gsz);
-    else if (NT == 768)
+    if (NT == 512) {
+        const int npair = (nzb + 1) / 2, nap = (n + 1) / 2;
+        const int g16 = xcd_group(bb * nap, npair, 150.0);
+        const long long s16 = (bb * nap + g16 - 1) / g16, P16 = kXcds / g16;
+        const dim3 grid16((unsigned)(kXcds * s16 * ((npair + P16 - 1) / P16)));
+        hipLaunchKernelGGL(radon_rows16_kernel, grid16, dim3(1024), 0, s, d_img, w, h, nz, na,
+                           bins, d_cos, d_sin, d_alist, d_sino, d_side, n, g16);
+    } else if (NT == 768)
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 776, 768, true>), grid, dim3(768), 0,
                            s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
                            n, gsz);
