@@ -47,8 +47,8 @@ FLOP_PER_SAMPLE_DEPOSIT = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows_kernel": 9.644e+09,
-    "bp_kernel": 6.459e+09,
+    "radon_rows16_kernel": 2.953e+10,
+    "bp_kernel": 6.493e+09,
 }
 
 
@@ -134,7 +134,7 @@ def cpu_baseline(vol_host: np.ndarray, na: int, ftype: int, target_s: float, run
 def dominant_roofline(n, na, kt, launches):
     """Roofline entry of the kernel with the largest time per step.
 
-    Candidates: the Radon row walk (radon_rows_kernel: one launch over the
+    Candidates: the Radon row walk (radon_rows16_kernel: one launch over the
     image for the angles with |sin| <= |cos|, one over the transposed image
     for the others) and the backprojection (bp_kernel). Both are FP32 VALU
     work (no MFMA, no HBM bound: a few % of HBM); achieved = algorithmic FLOP
@@ -143,7 +143,7 @@ def dominant_roofline(n, na, kt, launches):
     """
     samples = float(n) * n * n  # pixels x slices per angle
     cands = {
-        "radon_rows_kernel": (kt["walk_image"] + kt["walk_transpose"], 2 * max(launches, 1),
+        "radon_rows16_kernel": (kt["walk_image"] + kt["walk_transpose"], 2 * max(launches, 1),
                               FLOP_PER_SAMPLE * samples * na,
                               "Radon LDS row walk (image and transposed image)"),
         "bp_kernel": (kt["backprojection"], 1, FLOP_PER_UPDATE * samples * na,
