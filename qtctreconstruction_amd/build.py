@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract
           "-Wall", "-Wno-unused-command-line-argument", "-Wno-unknown-pragmas",
           "-I", os.path.join(os.path.dirname(PKG), "include")]
 
-SOURCES = ["phantom.hip", "radon.hip", "filter.hip", "backproject.hip", "pointcloud.hip",
+SOURCES = ["phantom.hip", "radon.hip", "filter.hip", "filter_wave.hip", "backproject.hip", "pointcloud.hip",
            "filter_gemm.hip", "filter_design.cpp", "metrics.hip", "fileio.cpp", "capi.cpp",
            "batch.cpp", "serve.cpp", "hostpath.cpp", "noalloc.hip"]
 

@@ -223,9 +223,7 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
         const ct_fbp_job& j = jobs[i];
         const Geom g = geometry(j.bins);
         const ct_hip_ctx::Trig t = get_trig(c, j.angles_deg, j.na);
-        float* H = get_filter(c, g.nfft, j.filter);
-        float2* tw = get_twiddles(c, g.nfft);
-        fj[i].a = make_filt_args(j.d_sinos, j.nz, j.na, j.bins, g.sq, g.pad, g.nfft, H, tw, filts[i]);
+        fj[i].a = filter_args(c, j.d_sinos, j.nz, j.na, j.bins, j.filter, filts[i]);
         BpArgs& p = bj[i].a;
         p.filt = filts[i];
         p.out = j.d_out;
@@ -253,16 +251,17 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
         s->h_btab[k].item_begin = bitems;
         bitems += (((jobs[i].nz + kZB - 1) / kZB + Z2 - 1) / Z2) * tiles[i];
     }
-    // Filter work lists, one per FFT length: a persistent launch reserves LDS
-    // for its largest transform, so lengths are not mixed in one launch.
-    std::map<int, std::vector<int>> classes;  // nfft -> jobs
-    for (int i = 0; i < nj; ++i) classes[fj[i].a.nfft].push_back(i);
-    need((int)classes.size() <= kMaxFftClasses, "fbpBatch: too many FFT lengths");
-    std::vector<int> cls_nfft, cls_first;
+    // Filter work lists, one per transform class (LDS FFT length, or wave
+    // form R): a persistent launch reserves LDS and registers for its class,
+    // so classes are not mixed in one launch.
+    std::map<int, std::vector<int>> classes;  // filter_class -> jobs
+    for (int i = 0; i < nj; ++i) classes[filter_class(fj[i].a)].push_back(i);
+    need((int)classes.size() <= kMaxFftClasses, "fbpBatch: too many row filter classes");
+    std::vector<int> cls_key, cls_first;
     int pos = 0, ci = 0;
     for (auto& kv : classes) {
         int items = 0;
-        cls_nfft.push_back(kv.first);
+        cls_key.push_back(kv.first);
         cls_first.push_back(pos);
         for (int i : kv.second) {
             s->h_ftab[pos] = fj[i];
@@ -288,7 +287,7 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
               "upload header");
     std::vector<int> fgrid(ci);
     for (int k = 0; k < ci; ++k) {
-        fgrid[k] = filter_persistent_grid(cls_nfft[k], c->device);
+        fgrid[k] = filter_persistent_grid(cls_key[k], c->device);
         need(fgrid[k] > 0, "fbpBatch: no occupancy for the persistent row filter");
     }
     const int bgrid = bp_persistent_grid(cfg, c->device);
@@ -299,7 +298,7 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
         // lengths, grids); where each class's jobs start is in the header
         for (int k = 0; k < ci; ++k)
             hip_check(launch_row_filter_persistent(s->ftab, s->hdr + 2 + 3 * k,
-                                                   s->counters + 1 + k, cls_nfft[k], fgrid[k],
+                                                   s->counters + 1 + k, cls_key[k], fgrid[k],
                                                    s->stream),
                       "persistent row filter");
         hip_check(launch_bp_persistent(cfg, s->btab, s->hdr, s->counters, bgrid, s->stream),
@@ -309,7 +308,7 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
     if (flags & CT_BATCH_GRAPH) {
         std::vector<int> key{cfg, bgrid};
         for (int k = 0; k < ci; ++k) {
-            key.push_back(cls_nfft[k]);
+            key.push_back(cls_key[k]);
             key.push_back(fgrid[k]);
         }
         auto it = s->graphs.find(key);

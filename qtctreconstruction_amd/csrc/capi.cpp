@@ -176,6 +176,58 @@ float2* get_twiddles(ct_hip_ctx* c, int nfft) {
     return t;
 }
 
+void wave_tables_host(const Geom& g, int filter, float2* tw, float* hs) {
+    std::vector<float> taps(g.nfft / 2 + 1);
+    serial_filter_taps(g.nfft, norm_filter(filter), taps.data());
+    taps[g.nfft / 2] = 0.0f;
+    for (int i = 0; i < kWaveHsFloats; ++i) hs[i] = 0.0f;
+    wave_filter_tables(g.bins, g.sq, g.pad, g.nfft, taps.data(), tw, hs);
+}
+
+// Tables of the wave-form row filter, made on the host in FP64 from the
+// reference's taps (FFT.cpp:87-120 semantics: Nyquist bin dropped).
+static float2* get_wave_tables(ct_hip_ctx* c, const Geom& g, int type) {
+    type = norm_filter(type);
+    const auto k = std::make_pair(g.bins, type);
+    auto it = c->wave_tabs.find(k);
+    if (it != c->wave_tabs.end()) return it->second;
+    std::vector<float2> tw(kWaveTabEntries);
+    std::vector<float> hs(kWaveHsFloats);
+    wave_tables_host(g, type, tw.data(), hs.data());
+    const size_t tw_bytes = sizeof(float2) * kWaveTabEntries;
+    float2* d = static_cast<float2*>(dmalloc(c, tw_bytes + sizeof(float) * kWaveHsFloats));
+    hip_check(hipMemcpyAsync(d, tw.data(), tw_bytes, hipMemcpyHostToDevice, c->stream),
+              "upload wave twiddles");
+    hip_check(hipMemcpyAsync(reinterpret_cast<char*>(d) + tw_bytes, hs.data(),
+                             sizeof(float) * kWaveHsFloats, hipMemcpyHostToDevice, c->stream),
+              "upload wave filter");
+    hip_check(hipStreamSynchronize(c->stream), "wave tables");
+    c->wave_tabs[k] = d;
+    return d;
+}
+
+FiltArgs filter_args(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int filter,
+                     float4* filt) {
+    const Geom g = geometry(bins);
+    const int r = wave_filter_r(bins, g.sq, g.pad, g.nfft);
+    if (r > 0) {
+        float2* t = get_wave_tables(c, g, filter);
+        const float* hs = reinterpret_cast<const float*>(t + kWaveTabEntries);
+        FiltArgs p = make_filt_args(d_sino, nz, na, bins, g.sq, g.pad, g.nfft, hs, t, filt);
+        p.wave_r = r;
+        return p;
+    }
+    check_fft(g);
+    return make_filt_args(d_sino, nz, na, bins, g.sq, g.pad, g.nfft, get_filter(c, g.nfft, filter),
+                          get_twiddles(c, g.nfft), filt);
+}
+
+hipError_t launch_filter(const FiltArgs& p, hipStream_t s) {
+    if (p.wave_r > 0) return launch_row_filter_wave(p, s);
+    return launch_row_filter_fft(p.sino, p.nz, p.na, p.bins, p.sq, p.pad, p.nfft, p.H, p.tw,
+                                 p.filt, s);
+}
+
 // SerialBackend.cpp:152-161
 Geom geometry(int bins) {
     Geom g;
@@ -303,11 +355,7 @@ void stage_filter(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, 
                   float4* filt) {
     const Geom g = geometry(bins);
     if (c->filter_backend == CT_FILTER_BACKEND_FFT_F32) {
-        check_fft(g);
-        const float* H = get_filter(c, g.nfft, filter);
-        const float2* tw = get_twiddles(c, g.nfft);
-        hip_check(launch_row_filter_fft(d_sino, nz, na, bins, g.sq, g.pad, g.nfft, H, tw, filt,
-                                        c->stream),
+        hip_check(launch_filter(filter_args(c, d_sino, nz, na, bins, filter, filt), c->stream),
                   "row filter");
         return;
     }
@@ -569,8 +617,10 @@ int ct_hip_clear_workspace(ct_hip_ctx* c) {
         batch_release(c);
         release_host_paths(c);
         for (auto& kv : c->twiddles) dfree(c, kv.second);
+        for (auto& kv : c->wave_tabs) dfree(c, kv.second);
         c->filters.clear();
         c->twiddles.clear();
+        c->wave_tabs.clear();
     });
 }
 

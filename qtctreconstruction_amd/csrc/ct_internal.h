@@ -128,9 +128,11 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
 struct FiltArgs {
     const float* sino;  // [nz][na][bins]
     int nz, na, bins, sq, pad, nfft, logn;
-    const float* H;     // nfft/2+1 taps
-    const float2* tw;   // nfft-1 twiddles, stage order (W_m^j at m/2-1+j)
+    const float* H;     // LDS form: nfft/2+1 taps; wave form: the short-transform filter
+    const float2* tw;   // LDS form: nfft-1 twiddles, stage order (W_m^j at m/2-1+j);
+                        // wave form: the wave twiddle table
     float4* filt;       // [nzb][na][sq] float4
+    int wave_r;         // 0: LDS FFT of nfft points; 1..3: wave form of 512 wave_r points
 };
 struct FiltJob {
     FiltArgs a;
@@ -140,16 +142,33 @@ struct FiltJob {
 FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, int pad, int nfft,
                         const float* d_H, const float2* d_tw, float4* d_filt);
 size_t fft_lds_bytes(int nfft);
-// One persistent launch per FFT length: every job in its list has nfft.
-int filter_persistent_grid(int nfft, int device);
+// One persistent launch per transform class: every job in its list has the
+// class cls (nfft of the LDS form, or -R of the wave form).
+int filter_persistent_grid(int cls, int device);
 // hdr = {first job index in d_jobs, njobs, total items}; counter must be 0
 // at launch. Everything batch-dependent is read from device memory.
 hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
-                                        int nfft, int grid, hipStream_t s);
+                                        int cls, int grid, hipStream_t s);
 // FilterDesign::createFilter(n, type): n/2 + 1 taps, the reference's FP32
 // arithmetic bit for bit (host; n a power of two >= 4)
 void serial_filter_taps(int n, int type, float* H);
 hipError_t launch_twiddles(int n, float2* d_tw, hipStream_t s);
+// Wave form of the row filter (filter_wave.hip): rows of up to kWaveM bins
+// whose padded transform has nfft >= 512 run as R sub-transforms of kWaveM
+// points (a short circular convolution of 512 R >= 2D + 1 points, D the
+// largest kernel offset the kept outputs meet). wave_filter_r gives R (0 when
+// the LDS form takes the row); the tables are made on the host in FP64 from
+// the reference's taps: tw holds kWaveTabEntries float2, hs kWaveHsFloats.
+constexpr int kWaveM = 512;
+constexpr int kWaveTabEntries = 1600;
+constexpr int kWaveHsFloats = 64 * 8 * 3;
+int wave_filter_r(int bins, int sq, int pad, int nfft);
+void wave_filter_tables(int bins, int sq, int pad, int nfft, const float* taps, float2* tw,
+                        float* hs);
+hipError_t launch_row_filter_wave(const FiltArgs& p, hipStream_t s);
+int filter_wave_persistent_grid(int r, int device);
+hipError_t launch_row_filter_wave_persistent(const FiltJob* d_jobs, const int* d_hdr,
+                                             int* d_counter, int r, int grid, hipStream_t s);
 // sino [nz][na][bins] -> filt [nzb][na][sq] float4 (4 slices interleaved)
 hipError_t launch_row_filter_fft(const float* d_sino, int nz, int na, int bins, int sq, int pad,
                                  int nfft, const float* d_H, const float2* d_tw, float4* d_filt,

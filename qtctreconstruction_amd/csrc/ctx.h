@@ -95,7 +95,9 @@ struct HostSlabs {
 
 // Largest number of volumes in one batch.
 constexpr int kMaxBatch = 256;
-// Distinct FFT lengths in one batch (64 .. 4096).
+// Distinct row filter classes in one batch: the LDS FFT lengths rows take
+// (64, 128, 256 for up to 90 bins; 2048, 4096 past 512 bins) and the three
+// wave-form lengths (512 R, R = 1..3, for the rest).
 constexpr int kMaxFftClasses = 8;
 
 // Launch state of one stream that runs batches (batch.cpp).
@@ -131,6 +133,10 @@ struct ct_hip_ctx {
     // cached device state
     std::map<std::pair<int, int>, float*> filters;  // (nfft, type) -> H[nfft/2+1]
     std::map<int, float2*> twiddles;                // nfft -> tw[nfft-1], stage order
+    // wave-form row filter tables, (bins, type) -> one allocation: twiddles
+    // (kWaveTabEntries float2), then the filter (kWaveHsFloats floats); bins
+    // <= kWaveM and six types bound it to 3072 entries of 19 KB
+    std::map<std::pair<int, int>, float2*> wave_tabs;
     struct Trig {
         float* cos = nullptr;
         float* sin = nullptr;
@@ -225,6 +231,17 @@ size_t volume_ws_per_slice(int w, int h, int na, bool sino_ws);
 ct_hip_ctx::Trig get_trig(ct_hip_ctx* c, const float* angles_deg, int na);
 float* get_filter(ct_hip_ctx* c, int nfft, int type);
 float2* get_twiddles(ct_hip_ctx* c, int nfft);
+// The row filter's arguments for one stack: the wave form (short transforms,
+// rows of up to kWaveM bins) or the LDS FFT form, with their cached tables.
+cthip::FiltArgs filter_args(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int filter,
+                     float4* filt);
+// Host copies of the wave form's tables (kWaveTabEntries float2, kWaveHsFloats
+// floats); R = wave_filter_r(...) > 0 for this geometry.
+void wave_tables_host(const Geom& g, int filter, float2* tw, float* hs);
+hipError_t launch_filter(const cthip::FiltArgs& p, hipStream_t s);
+// Work-list class of a row filter launch: nfft for the LDS form, -R for the
+// wave form (a persistent launch holds one class).
+inline int filter_class(const cthip::FiltArgs& p) { return p.wave_r > 0 ? -p.wave_r : p.nfft; }
 void bind_device(ct_hip_ctx* c);
 void batch_release(ct_hip_ctx* c);
 int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times);

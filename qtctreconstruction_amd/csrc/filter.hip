@@ -407,9 +407,11 @@ static int log2_of(int n) {
     return l;
 }
 
-// nfft: the FFT length of every job in the launch's list (64 .. kMaxFFT).
-int filter_persistent_grid(int nfft, int device) {
-    switch (log2_of(nfft)) {
+// cls: the transform class of every job in the launch's list: an FFT length
+// (64 .. kMaxFFT) of the LDS form, or -R of the wave form.
+int filter_persistent_grid(int cls, int device) {
+    if (cls < 0) return filter_wave_persistent_grid(-cls, device);
+    switch (log2_of(cls)) {
         case 6: return persistent_grid_n<6>(device);
         case 7: return persistent_grid_n<7>(device);
         case 8: return persistent_grid_n<8>(device);
@@ -422,8 +424,9 @@ int filter_persistent_grid(int nfft, int device) {
 }
 
 hipError_t launch_row_filter_persistent(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
-                                        int nfft, int grid, hipStream_t s) {
-    switch (log2_of(nfft)) {
+                                        int cls, int grid, hipStream_t s) {
+    if (cls < 0) return launch_row_filter_wave_persistent(d_jobs, d_hdr, d_counter, -cls, grid, s);
+    switch (log2_of(cls)) {
         case 6: return launch_persistent_n<6>(d_jobs, d_hdr, d_counter, grid, s);
         case 7: return launch_persistent_n<7>(d_jobs, d_hdr, d_counter, grid, s);
         case 8: return launch_persistent_n<8>(d_jobs, d_hdr, d_counter, grid, s);
@@ -456,6 +459,7 @@ FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, i
     p.H = d_H;
     p.tw = d_tw;
     p.filt = d_filt;
+    p.wave_r = 0;
     return p;
 }
 

@@ -2,8 +2,9 @@
 // device allocation, for callers that cannot allocate (code under HIP graph
 // capture, tight serving loops, harnesses that forbid hipMalloc).
 //
-// Its only scratch (the angle table, the filter taps, the FFT twiddles and
-// the filtered rows) lives in one file-scope array of kScratchBytes per GPU,
+// Its only scratch (the angle table, the filter taps and FFT twiddles of
+// either row filter form, and the filtered rows) lives in one file-scope
+// array of kScratchBytes per GPU,
 // which the device holds from the library's load to the process's end. The
 // filtered rows go through it in slabs of whole 4-slice z-blocks, so any
 // depth fits; a shape whose single z-block of filtered rows (na x sq x 16
@@ -28,13 +29,17 @@ namespace cthost {
 // at 720 angles x 725 padded bins), so 7 z-blocks per slab at that shape.
 constexpr size_t kScratchBytes = size_t(64) << 20;
 // Head of the array: the angle table (2 x kNoAllocMaxAngles floats), the
-// filter taps (kMaxFFT / 2 + 1 floats) and the twiddles (kMaxFFT - 1 float2),
-// each 256-byte aligned; the filtered rows take the rest.
+// LDS form's filter taps (kMaxFFT / 2 + 1 floats) and twiddles (kMaxFFT - 1
+// float2), the wave form's twiddles and filter (kWaveTabEntries float2, then
+// kWaveHsFloats floats), each 256-byte aligned; the filtered rows take the
+// rest.
 constexpr int kNoAllocMaxAngles = 8192;
 constexpr size_t kTrigOff = 0;
 constexpr size_t kTapsOff = kTrigOff + 2 * sizeof(float) * kNoAllocMaxAngles;
 constexpr size_t kTwOff = kTapsOff + (sizeof(float) * (kMaxFFT / 2 + 1) + 255) / 256 * 256;
-constexpr size_t kFiltOff = kTwOff + (sizeof(float2) * (kMaxFFT - 1) + 255) / 256 * 256;
+constexpr size_t kWaveOff = kTwOff + (sizeof(float2) * (kMaxFFT - 1) + 255) / 256 * 256;
+constexpr size_t kWaveBytes = sizeof(float2) * kWaveTabEntries + sizeof(float) * kWaveHsFloats;
+constexpr size_t kFiltOff = kWaveOff + (kWaveBytes + 255) / 256 * 256;
 static_assert(kFiltOff < kScratchBytes, "scratch head larger than the scratch");
 
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
@@ -100,24 +105,42 @@ static void from_sinos_noalloc(ct_hip_ctx* c, const float* d_sinos, int nz, int 
     float* d_H = reinterpret_cast<float*>(base + kTapsOff);
     float2* d_tw = reinterpret_cast<float2*>(base + kTwOff);
     float4* d_filt = reinterpret_cast<float4*>(base + kFiltOff);
-    // Serial's trig (SerialBackend.cpp:198-204) and taps with the Nyquist bin
-    // dropped (FFT.cpp:87-120), made on the host and copied in; the stream is
+    float2* d_wtw = reinterpret_cast<float2*>(base + kWaveOff);
+    float* d_whs = reinterpret_cast<float*>(d_wtw + kWaveTabEntries);
+    // Serial's trig (SerialBackend.cpp:198-204) and the row filter's tables
+    // (taps with the Nyquist bin dropped, FFT.cpp:87-120), made on the host
+    // as the allocating entry point makes them and copied in; the stream is
     // drained before return, so these host arrays outlive the copies.
     static thread_local float trig[2 * kNoAllocMaxAngles];
     static thread_local float taps[kMaxFFT / 2 + 1];
+    static thread_local float2 wtw[kWaveTabEntries];
+    static thread_local float whs[kWaveHsFloats];
     for (int a = 0; a < na; ++a) {
         const float th = deg2rad(angles_deg[a]);
         trig[a] = std::cos(th);
         trig[na + a] = std::sin(th);
     }
-    serial_filter_taps(g.nfft, norm_filter(filter), taps);
-    taps[g.nfft / 2] = 0.0f;
     hip_check(hipMemcpyAsync(d_cos, trig, sizeof(float) * 2 * na, hipMemcpyHostToDevice, c->stream),
               "noalloc: angle table");
-    hip_check(hipMemcpyAsync(d_H, taps, sizeof(float) * (g.nfft / 2 + 1), hipMemcpyHostToDevice,
-                             c->stream),
-              "noalloc: filter taps");
-    hip_check(launch_twiddles(g.nfft, d_tw, c->stream), "noalloc: twiddles");
+    const int wave_r = wave_filter_r(bins, g.sq, g.pad, g.nfft);
+    FiltArgs fa;
+    if (wave_r > 0) {
+        wave_tables_host(g, filter, wtw, whs);
+        hip_check(hipMemcpyAsync(d_wtw, wtw, sizeof(wtw), hipMemcpyHostToDevice, c->stream),
+                  "noalloc: wave twiddles");
+        hip_check(hipMemcpyAsync(d_whs, whs, sizeof(whs), hipMemcpyHostToDevice, c->stream),
+                  "noalloc: wave filter");
+        fa = make_filt_args(d_sinos, nz, na, bins, g.sq, g.pad, g.nfft, d_whs, d_wtw, d_filt);
+        fa.wave_r = wave_r;
+    } else {
+        serial_filter_taps(g.nfft, norm_filter(filter), taps);
+        taps[g.nfft / 2] = 0.0f;
+        hip_check(hipMemcpyAsync(d_H, taps, sizeof(float) * (g.nfft / 2 + 1),
+                                 hipMemcpyHostToDevice, c->stream),
+                  "noalloc: filter taps");
+        hip_check(launch_twiddles(g.nfft, d_tw, c->stream), "noalloc: twiddles");
+        fa = make_filt_args(d_sinos, nz, na, bins, g.sq, g.pad, g.nfft, d_H, d_tw, d_filt);
+    }
     // the tuned configuration when this (N, na) class has been tuned, else the
     // context's: every configuration gives the same bits, and tuning allocates
     auto it = c->tuned.find({bins, na});
@@ -127,9 +150,10 @@ static void from_sinos_noalloc(ct_hip_ctx* c, const float* d_sinos, int nz, int 
     const int slab = (int)std::min<size_t>(zbs * kZB, (size_t)nz);
     for (int z0 = 0; z0 < nz; z0 += slab) {
         const int n = std::min(slab, nz - z0);
-        hip_check(launch_row_filter_fft(d_sinos + (size_t)z0 * na * bins, n, na, bins, g.sq, g.pad,
-                                        g.nfft, d_H, d_tw, d_filt, c->stream),
-                  "noalloc: row filter");
+        FiltArgs slab = fa;
+        slab.sino = d_sinos + (size_t)z0 * na * bins;
+        slab.nz = n;
+        hip_check(launch_filter(slab, c->stream), "noalloc: row filter");
         hip_check(launch_backproject(d_filt, n, na, g.sq, bins, d_cos, d_sin, scale, nullptr,
                                      d_out + (size_t)z0 * bins * bins, c->stream, cfg),
                   "noalloc: backprojection");

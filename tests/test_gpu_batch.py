@@ -95,8 +95,8 @@ def test_graph_replay_with_changing_class_composition(O):
     memory, not in the captured kernel arguments)."""
     be = ct.HipBackend(0)
     ctx = be.ctx
-    small = [(40, 3, 24), (48, 2, 30), (44, 4, 18)]   # nfft 128
-    large = [(96, 2, 36), (100, 3, 20)]               # nfft 512 / 512
+    small = [(40, 3, 24), (48, 2, 30), (44, 4, 18)]   # LDS FFT of 128 points
+    large = [(96, 2, 36), (100, 3, 20)]               # wave form, one 512-point transform
     pool = {}
     for k, (N, nz, na) in enumerate(small + large):
         s = _make_sinos(O, 300 + k, N, nz, na)
@@ -112,6 +112,31 @@ def test_graph_replay_with_changing_class_composition(O):
             s, ds, do = pool[(N, nz, na)]
             ref = _alone(be, s, O.serial_angles(na), ct.FilterType.Ramp)
             assert np.array_equal(do.download((nz, N, N)), ref), (comp, N)
+
+
+def test_batch_every_filter_class(O):
+    """One batch holding every row filter class: the LDS FFT below 91 bins
+    and past 512, and the wave form with 1, 2 and 3 sub-transforms of 512
+    points (one persistent launch per class). Each volume equals its
+    standalone reconstruction bit for bit, and its first slice matches the
+    Serial oracle."""
+    be = ct.HipBackend(0)
+    ctx = be.ctx
+    shapes = [(40, 3, 12), (96, 2, 10), (300, 2, 8), (512, 1, 6), (600, 2, 5)]
+    jobs, keep = [], []
+    for k, (N, nz, na) in enumerate(shapes):
+        s = _make_sinos(O, 500 + k, N, nz, na)
+        ds, do = ctx.alloc(s.nbytes).upload(s), ctx.alloc(4 * nz * N * N)
+        jobs.append((ds, do, O.serial_angles(na), nz, na, N, ct.FilterType.Hann))
+        keep.append((s, do, N, nz, na))
+    for flags in (0, ct.CT_BATCH_GRAPH):
+        ctx.fbp_batch(jobs, flags)
+        for (s, do, N, nz, na) in keep:
+            got = do.download((nz, N, N))
+            ref = _alone(be, s, O.serial_angles(na), ct.FilterType.Hann)
+            assert np.array_equal(got, ref), (flags, N)
+            ser = O.reconstruct_slice(s[0], O.serial_angles(na), N, O.HANN)
+            assert O.rel_l2(got[0], ser) <= 1e-5, (flags, N)
 
 
 def test_tune_cache_file_is_read_by_every_context(tmp_path, monkeypatch):
