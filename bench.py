@@ -3,7 +3,8 @@
 A step is one reconstructVolume over device-resident input: per-slice
 min/max + normalisation, ray-driven Radon (720 angles x 512 bins; the
 LDS-staged row walk over the image and over its transpose, each timed with
-HIP events), the Shepp-Logan FFT row filter (sq 725, nfft 2048) and
+HIP events), the Shepp-Logan row filter (sq 725: the 2048-point padded
+convolution as short transforms of 3 x 512 points) and
 voxel-driven backprojection with denormalisation, for a 512^3 phantom of 24
 seeded ellipsoids (the plain-FP32 single-GPU configuration). value =
 voxel-updates/s = 512^3 * 720 / step time.
@@ -168,7 +169,7 @@ def dominant_roofline(n, na, kt, launches):
 
 def stage_rooflines(n, na, kt, n_row):
     """Every stage of a step against the roofline that binds it: the streaming
-    stages (min/max, normalisation, FFT row filter) against HBM with their
+    stages (min/max, normalisation, row filter) against HBM with their
     compulsory bytes, the Radon walks and the backprojection against FP32
     (with their compulsory HBM bytes beside). Times: HIP events per kernel."""
     sq = int(np.ceil(np.sqrt(2.0) * n))
@@ -191,7 +192,7 @@ def stage_rooflines(n, na, kt, n_row):
                                  kt["walk_image"]),
         "radon_walk_transpose": fp32(FLOP_PER_SAMPLE * vox * (na - n_row),
                                      copies / 2 + 4.0 * n * (na - n_row) * n, kt["walk_transpose"]),
-        "row_filter_fft": hbm(4.0 * n * na * n + 4.0 * n * na * sq, kt["row_filter"]),
+        "row_filter": hbm(4.0 * n * na * n + 4.0 * n * na * sq, kt["row_filter"]),
         "backprojection": fp32(FLOP_PER_UPDATE * vox * na, 4.0 * n * na * sq + 4 * vox,
                                kt["backprojection"]),
     }
