@@ -318,6 +318,16 @@ def batched_form(ctx, seed=4):
         ctx.fbp_batch(bjobs, flags)
     t_batch = (time.perf_counter() - t0) / reps
 
+    # the same batch with the split-BF16 Toeplitz GEMM as the row filter (one
+    # MFMA launch per volume ahead of the captured backprojection)
+    ctx.set_filter_backend(ct.FilterBackend.GEMM_SPLIT_BF16)
+    ctx.fbp_batch(bjobs, flags)  # designs the Toeplitz tables, captures
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.fbp_batch(bjobs, flags)
+    t_split = (time.perf_counter() - t0) / reps
+    ctx.set_filter_backend(ct.FilterBackend.FFT_F32)
+
     def one_by_one():
         for (d_s, d_o, ang, nz, na, B, f, _) in jobs:
             ctx.reconstruct_from_sinograms_dev(d_s, nz, na, B, ang, f, d_o)
@@ -334,6 +344,8 @@ def batched_form(ctx, seed=4):
             "batched_voxel_updates_per_s": updates / t_batch,
             "one_by_one_ms": t_seq * 1e3, "one_by_one_voxel_updates_per_s": updates / t_seq,
             "batched_vs_one_by_one": t_seq / t_batch,
+            "batched_split_bf16_ms": t_split * 1e3,
+            "batched_split_bf16_voxel_updates_per_s": updates / t_split,
             "bins": "ceil(sqrt2 N) + 2; output B x B (consumer path)"}
 
 

@@ -210,10 +210,12 @@ typedef struct {
 enum { CT_BATCH_GRAPH = 1,     /* capture the launches once in a hipGraph and replay it */
        CT_BATCH_AUTOTUNE = 2   /* pick the backprojection tiles with the autotuner */ };
 
-/* Batched consumer path: the row filters of every volume run in one
- * persistent launch and the backprojections in a second, both fed from work
- * lists; there is no reference counterpart (the reference loops per slice,
- * BackendFactory.cpp:17-60). Waits for completion. */
+/* Batched consumer path: the row filters of every volume run in persistent
+ * launches (one per transform class) and the backprojections in one more,
+ * all fed from work lists; with a Toeplitz-GEMM filter backend set on the
+ * context (ct_hip_set_filter_backend) the filter is one MFMA launch per
+ * volume instead. There is no reference counterpart (the reference loops per
+ * slice, BackendFactory.cpp:17-60). Waits for completion. */
 int ct_hip_fbp_batch_dev(ct_hip_ctx* ctx, const ct_fbp_job* jobs, int njobs, int flags);
 
 /* Backprojection tile configurations (voxel tile edge, angles per LDS
@@ -244,6 +246,14 @@ int ct_hip_server_create(int device, size_t pool_bytes, int max_batch, int slots
 void ct_hip_server_destroy(ct_hip_server* srv);
 /* CT_BATCH_* flags for the server's batches (default graph + autotune). */
 int ct_hip_server_set_flags(ct_hip_server* srv, int flags);
+/* The row filter backend of the server's batches (CT_FILTER_BACKEND_*;
+ * default FFT_F32), for the batches formed after the call. With a
+ * Toeplitz-GEMM backend each job's filter is one MFMA launch (split BF16
+ * stays within 1e-4 of FP64) and its result equals a standalone
+ * reconstructVolumeFromSinograms on a context with the same backend.
+ * No counterpart in the reference: its backends pick their FFT at build
+ * time (IFFTBackend). */
+int ct_hip_server_set_filter_backend(ct_hip_server* srv, int backend);
 /* Admission policy:
  *   CT_SERVE_FIFO        submission order;
  *   CT_SERVE_SIZE_AGED   (default) ascending estimated device time minus

@@ -32,8 +32,11 @@
  *   ct_ph_*                FP64 membership of one ellipsoid; volume closed form;
  *       additivity; determinism
  *   ct_o64_*               the closed forms above at tight tolerance
- *   ct_omp_*               timing baseline only: parity unpinned (its output
- *       is not a parity target; its reference files do not build here)
+ *   ct_omp_*               the timed OpenMP baseline, pinned by its own
+ *       conventions' closed forms: theta = 0 column sums with the (int)
+ *       truncated bin, theta = 90 row reversal (-sin, no y-flip), negative-u
+ *       truncation with mass kept, a disk reconstructing unflipped at its
+ *       centre with amplitude ~1, linearity (tests/test_oracle.py)
  */
 #ifndef CT_ORACLE_H
 #define CT_ORACLE_H

@@ -137,6 +137,7 @@ def load_library(path: str = LIB_PATH):
         "ct_hip_server_create": (i, [i, sz, i, i, ctypes.POINTER(_vp)]),
         "ct_hip_server_destroy": (None, [_vp]),
         "ct_hip_server_set_flags": (i, [_vp, i]),
+        "ct_hip_server_set_filter_backend": (i, [_vp, i]),
         "ct_hip_server_set_policy": (i, [_vp, i, ctypes.c_double, ctypes.c_double]),
         "ct_hip_server_submit": (i, [_vp, ctypes.POINTER(FbpJob), ctypes.POINTER(ctypes.c_uint64)]),
         "ct_hip_server_wait": (i, [_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64),
@@ -184,7 +185,8 @@ def exported_symbols() -> List[str]:
         "ct_hip_design_filter", "ct_hip_set_filter_backend", "ct_hip_filter_sinograms",
         "ct_hip_fbp_batch_dev", "ct_hip_bp_config_count", "ct_hip_bp_config", "ct_hip_autotune",
         "ct_hip_set_bp_config", "ct_hip_server_create", "ct_hip_server_destroy",
-        "ct_hip_server_set_flags", "ct_hip_server_set_policy", "ct_hip_server_submit",
+        "ct_hip_server_set_flags", "ct_hip_server_set_filter_backend",
+        "ct_hip_server_set_policy", "ct_hip_server_submit",
         "ct_hip_server_wait",
         "ct_hip_server_download", "ct_hip_server_release", "ct_hip_server_stats",
         "ct_hip_server_last_error", "ct_hip_radon_dev", "ct_hip_noisy_log_dev",
@@ -441,6 +443,13 @@ class Server:
 
     def set_flags(self, flags: int):
         self.lib.ct_hip_server_set_flags(self.handle, int(flags))
+
+    def set_filter_backend(self, backend: int):
+        """Row filter backend (FilterBackend) of the batches formed from now on;
+        a Toeplitz-GEMM backend filters each job with one MFMA launch."""
+        rc = self.lib.ct_hip_server_set_filter_backend(self.handle, int(backend))
+        if rc != CT_OK:
+            raise BackendError(rc, f"set_filter_backend: unknown backend {int(backend)}")
 
     def set_policy(self, policy: int = 1, batch_cap_ms: float = 5.0, age_weight: float = 0.25):
         """Admission order: SERVE_FIFO, or SERVE_SIZE_AGED (default) = ascending

@@ -62,6 +62,7 @@ void slot_release(ct_hip_ctx* c, BatchSlot* s) {
     dfree(c, s->btab);
     dfree(c, s->hdr);
     dfree(c, s->counters);
+    release(c, s->rmax);
     (void)hipHostFree(s->h_ftab);
     (void)hipHostFree(s->h_btab);
     (void)hipHostFree(s->h_hdr);
@@ -190,23 +191,30 @@ size_t filt_elems(const ct_fbp_job& j) {
 
 size_t job_filt_bytes(const ct_fbp_job& j) { return sizeof(float4) * filt_elems(j); }
 
-// Builds the two work lists for jobs[0..nj) (filtered sinograms go to
-// filts[i]) and enqueues counter reset + persistent filter + persistent
-// backprojection on the slot's stream, as a graph replay when flags ask for
-// it. Does not wait. The caller holds c->mu.
+// Builds the work lists for jobs[0..nj) (filtered sinograms go to filts[i])
+// and enqueues, on the slot's stream, the row filter (persistent launches,
+// one per transform class, for the FFT backend; one Toeplitz-GEMM launch per
+// job for the MFMA backends) and the persistent backprojection, with the
+// counter reset and the persistent launches as a graph replay when flags ask
+// for it. Does not wait. The caller holds c->mu.
 void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, float4* const* filts,
                    int flags) {
     need(jobs != nullptr && nj > 0, "fbpBatch: no jobs");
     need(nj <= kMaxBatch, "fbpBatch: too many jobs in one batch");
     trim_trig(c);  // before this batch's tables are looked up, never between
+    // The context's row filter backend: the FFT forms run as persistent
+    // launches over the batch's rows; a Toeplitz-GEMM form (MFMA) runs one
+    // launch per job, ahead of the captured backprojection.
+    const bool gemm = c->filter_backend != CT_FILTER_BACKEND_FFT_F32;
     long long work_best = -1;
+    size_t max_rows = 0;
     int dom = 0;
     for (int i = 0; i < nj; ++i) {
         const ct_fbp_job& j = jobs[i];
         need(j.d_sinos && j.d_out && j.angles_deg, "fbpBatch: null pointer in job");
         need(j.nz > 0 && j.na > 0 && j.bins > 0, "fbpBatch: empty job");
-        const Geom g = geometry(j.bins);
-        check_fft(g);
+        check_fbp(c, j.bins);
+        max_rows = std::max(max_rows, (size_t)j.nz * j.na);
         const long long work = (long long)j.bins * j.bins * j.nz * j.na;
         if (work > work_best) {
             work_best = work;
@@ -223,7 +231,8 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
         const ct_fbp_job& j = jobs[i];
         const Geom g = geometry(j.bins);
         const ct_hip_ctx::Trig t = get_trig(c, j.angles_deg, j.na);
-        fj[i].a = filter_args(c, j.d_sinos, j.nz, j.na, j.bins, j.filter, filts[i]);
+        if (gemm) (void)get_toeplitz(c, g, j.filter);  // made on the context stream
+        else fj[i].a = filter_args(c, j.d_sinos, j.nz, j.na, j.bins, j.filter, filts[i]);
         BpArgs& p = bj[i].a;
         p.filt = filts[i];
         p.out = j.d_out;
@@ -255,7 +264,8 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
     // form R): a persistent launch reserves LDS and registers for its class,
     // so classes are not mixed in one launch.
     std::map<int, std::vector<int>> classes;  // filter_class -> jobs
-    for (int i = 0; i < nj; ++i) classes[filter_class(fj[i].a)].push_back(i);
+    if (!gemm)
+        for (int i = 0; i < nj; ++i) classes[filter_class(fj[i].a)].push_back(i);
     need((int)classes.size() <= kMaxFftClasses, "fbpBatch: too many row filter classes");
     std::vector<int> cls_key, cls_first;
     int pos = 0, ci = 0;
@@ -276,9 +286,10 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
     }
     s->h_hdr[0] = nj;
     s->h_hdr[1] = bitems;
-    hip_check(hipMemcpyAsync(s->ftab, s->h_ftab, sizeof(FiltJob) * nj, hipMemcpyHostToDevice,
-                             s->stream),
-              "upload filter work list");
+    if (!gemm)
+        hip_check(hipMemcpyAsync(s->ftab, s->h_ftab, sizeof(FiltJob) * nj, hipMemcpyHostToDevice,
+                                 s->stream),
+                  "upload filter work list");
     hip_check(hipMemcpyAsync(s->btab, s->h_btab, sizeof(BpJob) * nj, hipMemcpyHostToDevice,
                              s->stream),
               "upload backprojection work list");
@@ -305,8 +316,19 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
                   "persistent backprojection");
     };
     hip_check(hipEventRecord(s->ev_start, s->stream), "event");
+    if (gemm) {
+        // one buffer of row maxima for the FP8 form, sized for the largest
+        // job before any launch (jobs use it one after another on the stream)
+        float* rmax = c->filter_backend == CT_FILTER_BACKEND_GEMM_FP8
+                          ? ensure<float>(c, s->rmax, sizeof(float) * max_rows)
+                          : nullptr;
+        for (int i = 0; i < nj; ++i) {
+            const ct_fbp_job& j = jobs[i];
+            launch_gemm_filter(c, j.d_sinos, j.nz, j.na, j.bins, j.filter, filts[i], rmax, s->stream);
+        }
+    }
     if (flags & CT_BATCH_GRAPH) {
-        std::vector<int> key{cfg, bgrid};
+        std::vector<int> key{cfg, bgrid, gemm ? 1 : 0};
         for (int k = 0; k < ci; ++k) {
             key.push_back(cls_key[k]);
             key.push_back(fgrid[k]);

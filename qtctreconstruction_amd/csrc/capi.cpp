@@ -350,24 +350,31 @@ const ct_hip_ctx::Toep& get_toeplitz(ct_hip_ctx* c, const Geom& g, int type) {
     return c->toeplitz[key] = t;
 }
 
+void launch_gemm_filter(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int filter,
+                        float4* filt, float* rmax, hipStream_t stream) {
+    const Geom g = geometry(bins);
+    const ct_hip_ctx::Toep& t = get_toeplitz(c, g, filter);
+    if (c->filter_backend == CT_FILTER_BACKEND_GEMM_FP8) {
+        need(rmax != nullptr, "FP8 filter: no row-maximum buffer");
+        hip_check(launch_row_absmax(d_sino, (size_t)nz * na, bins, rmax, stream), "row absmax");
+    }
+    hip_check(launch_toeplitz_gemm(c->filter_backend, d_sino, nz, na, bins, g.sq, t.Np, t.Kp, t.hi,
+                                   t.lo, t.f8, rmax, t.tmax, filt, stream),
+              "toeplitz gemm");
+}
+
 // row filter of nz sinograms [nz][na][bins] -> filt [nz/4][na][sq][4]
 void stage_filter(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int filter,
                   float4* filt) {
-    const Geom g = geometry(bins);
     if (c->filter_backend == CT_FILTER_BACKEND_FFT_F32) {
         hip_check(launch_filter(filter_args(c, d_sino, nz, na, bins, filter, filt), c->stream),
                   "row filter");
         return;
     }
-    const ct_hip_ctx::Toep& t = get_toeplitz(c, g, filter);
-    float* rmax = nullptr;
-    if (c->filter_backend == CT_FILTER_BACKEND_GEMM_FP8) {
-        rmax = ensure<float>(c, c->rmax, sizeof(float) * (size_t)nz * na);
-        hip_check(launch_row_absmax(d_sino, (size_t)nz * na, bins, rmax, c->stream), "row absmax");
-    }
-    hip_check(launch_toeplitz_gemm(c->filter_backend, d_sino, nz, na, bins, g.sq, t.Np, t.Kp, t.hi,
-                                   t.lo, t.f8, rmax, t.tmax, filt, c->stream),
-              "toeplitz gemm");
+    float* rmax = c->filter_backend == CT_FILTER_BACKEND_GEMM_FP8
+                      ? ensure<float>(c, c->rmax, sizeof(float) * (size_t)nz * na)
+                      : nullptr;
+    launch_gemm_filter(c, d_sino, nz, na, bins, filter, filt, rmax, c->stream);
 }
 
 // Backprojection tile configuration for (N, na): the fixed one, or the

@@ -112,6 +112,7 @@ struct BatchSlot {
     cthip::FiltJob* h_ftab = nullptr;  // pinned staging
     cthip::BpJob* h_btab = nullptr;
     int* h_hdr = nullptr;
+    Buf rmax;  // per-row maxima of the FP8 GEMM filter form (grow-only)
     std::map<std::vector<int>, hipGraphExec_t> graphs;  // keyed by the launch shape
     ~BatchSlot();
 };
@@ -239,6 +240,13 @@ cthip::FiltArgs filter_args(ct_hip_ctx* c, const float* d_sino, int nz, int na, 
 // floats); R = wave_filter_r(...) > 0 for this geometry.
 void wave_tables_host(const Geom& g, int filter, float2* tw, float* hs);
 hipError_t launch_filter(const cthip::FiltArgs& p, hipStream_t s);
+// Toeplitz tables of the GEMM row filter forms, cached per (nfft, type, bins)
+// and made on the context stream (which the call drains).
+const ct_hip_ctx::Toep& get_toeplitz(ct_hip_ctx* c, const Geom& g, int type);
+// The context backend's Toeplitz-GEMM row filter of one stack on `stream`;
+// the FP8 form needs nz * na floats of row maxima in rmax.
+void launch_gemm_filter(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int filter,
+                        float4* filt, float* rmax, hipStream_t stream);
 // Work-list class of a row filter launch: nfft for the LDS form, -R for the
 // wave form (a persistent launch holds one class).
 inline int filter_class(const cthip::FiltArgs& p) { return p.wave_r > 0 ? -p.wave_r : p.nfft; }

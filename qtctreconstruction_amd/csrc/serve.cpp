@@ -384,6 +384,13 @@ int ct_hip_server_set_flags(ct_hip_server* s, int flags) {
     return CT_OK;
 }
 
+int ct_hip_server_set_filter_backend(ct_hip_server* s, int backend) {
+    if (!s) return CT_ERR_INVALID;
+    // the worker forms batches under the context's lock, so a change applies
+    // to the batches formed after it returns
+    return ct_hip_set_filter_backend(s->ctx, backend);
+}
+
 int ct_hip_server_set_policy(ct_hip_server* s, int policy, double batch_cap_ms,
                              double age_weight) {
     if (!s || (policy != CT_SERVE_FIFO && policy != CT_SERVE_SIZE_AGED)) return CT_ERR_INVALID;

@@ -90,6 +90,41 @@ def test_alone_vs_mix_bit_identical(inputs, O):
             assert O.rel_l2(a[z], O.reconstruct_slice(j["host"][z], j["ang"], j["N"], int(j["f"]))) <= 1e-5
 
 
+def test_split_bf16_filter_backend(inputs, O):
+    """set_filter_backend(GEMM_SPLIT_BF16): each job's result equals a
+    standalone reconstructVolumeFromSinograms on a context with the same
+    backend bit for bit, and stays within 1e-4 of the FP64 oracle; the
+    default FFT backend comes back after the switch back."""
+    ctx, jobs = inputs
+    srv = ct.Server(pool_bytes=256 << 20, max_batch=8, slots=2)
+    be = ct.HipBackend(0)
+    be.ctx.set_filter_backend(ct.FilterBackend.GEMM_SPLIT_BF16)
+    srv.set_filter_backend(ct.FilterBackend.GEMM_SPLIT_BF16)
+    tickets = [srv.submit(j["dev"], j["ang"], j["nz"], j["na"], j["N"], j["f"]) for j in jobs]
+    done = {}
+    while len(done) < len(tickets):
+        r = srv.wait(30000)
+        assert r is not None
+        done[r[0]] = True
+    for t, j in zip(tickets, jobs):
+        got = srv.download(t, (j["nz"], j["N"], j["N"]))
+        srv.release(t)
+        ref = be.reconstructVolumeFromSinograms(j["host"], j["ang"],
+                                                ct.ReconstructionParams(j["f"], j["na"]))
+        assert np.array_equal(got, ref), j["N"]
+        ref64 = O.reconstruct_slice64(j["host"][0], j["ang"], j["N"], int(j["f"]))
+        assert O.rel_l2(got[0], ref64) <= 1e-4, j["N"]
+    srv.set_filter_backend(ct.FilterBackend.FFT_F32)
+    fft = _run_alone(srv, jobs[0])
+    be.ctx.set_filter_backend(ct.FilterBackend.FFT_F32)
+    ref = be.reconstructVolumeFromSinograms(jobs[0]["host"], jobs[0]["ang"],
+                                            ct.ReconstructionParams(jobs[0]["f"], jobs[0]["na"]))
+    assert np.array_equal(fft, ref)
+    with pytest.raises(ct.BackendError):
+        srv.set_filter_backend(9)
+    srv.close()
+
+
 def test_admission_by_budget_and_overlap(inputs):
     ctx, jobs = inputs
     j = jobs[3]  # 128^2 x 3 slices x 120 angles
