@@ -63,7 +63,8 @@ DRIVER = os.path.join(PKG, "hip_backend_driver")
 
 def build_driver() -> str:
     """The C++ IReconstructionBackend adapter + its driver (host code only, g++)."""
-    srcs = [os.path.join(HOST, f) for f in ("HipBackend.cpp", "BackendFactory.cpp", "FileIO.cpp",
+    srcs = [os.path.join(HOST, f) for f in ("HipBackend.cpp", "HipVolumeGenerator.cpp",
+                                             "BackendFactory.cpp", "FileIO.cpp",
                                              "hip_backend_driver.cpp")]
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", DRIVER + ".tmp", *srcs,
            "-L", PKG, "-lct_hip", "-Wl,-rpath,$ORIGIN"]

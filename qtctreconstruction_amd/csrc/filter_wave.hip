@@ -29,8 +29,8 @@
 // in registers. No workgroup barrier is needed: a wavefront's LDS
 // instructions execute in order, and a wavefront-scope fence keeps the
 // compiler from moving them across a transpose. Output is written into the
-// z-innermost float4 layout the backprojection reads: the block's two
-// wavefronts write the x,y and z,w halves of the same bins.
+// z-innermost float4 layout the backprojection reads: the two wavefronts of
+// a z-block write the x,y and z,w halves of the same bins.
 #include "ct_internal.h"
 
 #include <algorithm>
@@ -46,7 +46,6 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 typedef const float __attribute__((address_space(1))) g_cfloat;
 typedef const f2v __attribute__((address_space(1))) g_cfloat2;
 typedef const f4v __attribute__((address_space(1))) g_cfloat4;
-typedef f2v __attribute__((address_space(1))) g_float2;
 
 constexpr int kWS = 72;           // LDS slots per transpose row group (64 + 8)
 constexpr int kWSlots = 8 * kWS;  // one wavefront's transpose area, float2 slots
@@ -58,10 +57,6 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
     return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
-}
-__device__ __forceinline__ float2 ld2(g_cfloat2* p) {
-    const f2v v = *p;
-    return make_float2(v.x, v.y);
 }
 
 // a * W_8^j (W_8 = exp(-2 pi i / 8)), or a * W_8^-j for INV.

@@ -2,11 +2,17 @@
 // interface obtained from BackendFactory, the way the reference's controller
 // does (CtReconstructionController.cpp:1096-1146, 984-1003, 629-634), and
 // writes every result as raw float32 files for the parity test. The file
-// formats (FileIO.h) and the device metrics are driven the same way.
+// formats (FileIO.h) and the device metrics are driven the same way, and,
+// given a seed and an ellipsoid count, the phantom through IVolumeGenerator
+// as Generator3D::generateBrainHU calls it (Generator3D.cpp:41-44).
 //
 // usage: hip_backend_driver <volume.f32> <w> <h> <d> <num_angles> <filter> <threshold> <outdir>
+//                           [<phantom seed> <ellipsoids>]
 #include "FileIO.h"
 #include "HipBackend.h"
+#include "HipVolumeGenerator.h"
+
+#include <memory>
 
 #include <cstdio>
 #include <cstdlib>
@@ -23,8 +29,8 @@ static void dump(const std::string& path, const float* p, size_t n) {
 }
 
 int main(int argc, char** argv) {
-    if (argc != 9) {
-        std::cerr << "usage: " << argv[0] << " vol.f32 w h d na filter threshold outdir\n";
+    if (argc != 9 && argc != 11) {
+        std::cerr << "usage: " << argv[0] << " vol.f32 w h d na filter threshold outdir [seed n_ell]\n";
         return 2;
     }
     const size_t w = std::strtoul(argv[2], nullptr, 10), h = std::strtoul(argv[3], nullptr, 10),
@@ -121,6 +127,20 @@ int main(int argc, char** argv) {
         meta << default_calls << "\n";
         meta << m.mse << "\n" << m.mae << "\n" << m.psnr << "\n" << m.max_abs << "\n" << m.vmin
              << "\n" << m.vmax << "\n" << m.count << "\n";
+        if (argc == 11) {
+            std::unique_ptr<IVolumeGenerator> gen = std::make_unique<HipVolumeGenerator>();
+            Generator3D::Params gp;
+            gp.shape = {w, h, d};
+            gp.seed = (uint32_t)std::strtoul(argv[9], nullptr, 10);
+            gp.num_ellipsoids = std::strtoul(argv[10], nullptr, 10);
+            Volume ph = gen->generateBrainHU(gp);
+            dump(out + "/phantom.f32", ph.data.data(), ph.data.size());
+            meta << gen->name() << "\n"
+                 << gen->isAvailable() << "\n"
+                 << ph.width << " " << ph.height << " " << ph.depth << "\n"
+                 << ph.x_coords.front() << " " << ph.x_coords.back() << " " << ph.z_coords.size()
+                 << "\n";
+        }
     } catch (const std::exception& ex) {
         std::cerr << "exception: " << ex.what() << "\n";
         return 5;

@@ -415,9 +415,18 @@ def test_cpp_hip_backend_through_the_interface(O, tmp_path):
     vol = O.phantom(404, 10, w, h, d)
     vol.tofile(tmp_path / "vol.f32")
     r = subprocess.run([drv, str(tmp_path / "vol.f32"), str(w), str(h), str(d), str(na), str(filt),
-                        str(thr), str(tmp_path)], capture_output=True, text=True, timeout=120)
+                        str(thr), str(tmp_path), "404", "10"], capture_output=True, text=True,
+                       timeout=120)
     assert r.returncode == 0, r.stderr
     meta = (tmp_path / "meta.txt").read_text().split("\n")
+    # the phantom through IVolumeGenerator::generateBrainHU (HipVolumeGenerator):
+    # the same voxels as the oracle's phantom of that seed and count, bit for
+    # bit, with the reference's coordinate axes (linspace(-50, 50, n) mm)
+    assert meta[16] == "GPU (HIP, MI355X)" and meta[17] == "1"
+    assert meta[18].split() == [str(w), str(h), str(d)]
+    x0, x1, nzc = meta[19].split()
+    assert float(x0) == -50.0 and float(x1) == 50.0 and int(nzc) == d
+    assert _bits_equal(np.fromfile(tmp_path / "phantom.f32", np.float32).reshape(d, h, w), vol)
     assert meta[0] == "HIP (MI355X)" and int(meta[1]) == d and meta[3] == "1"
     sino0 = np.fromfile(tmp_path / "sino0.f32", np.float32).reshape(na, w)
     ref_s, _ = O.compute_sinogram(vol[0], na, w)
