@@ -699,6 +699,21 @@ def main():
         d_vol.free()
         d_out.free()
         extra["served_form"] = served_form(ctx, args.serve_jobs)
+    # The forms against the plain form's throughput (this line's value: the
+    # whole reconstructVolume pipeline, Radon included; the consumer-path forms
+    # start from sinograms, so they do less work per voxel-update) and, in
+    # their own fields, against plain per-job FBP on the same jobs.
+    vs_plain = {}
+    if "batched_form" in extra:
+        bf = extra["batched_form"]
+        vs_plain["batched"] = bf["batched_voxel_updates_per_s"] / value
+        vs_plain["batched_split_bf16"] = bf["batched_split_bf16_voxel_updates_per_s"] / value
+    if "served_form" in extra:
+        sf = extra["served_form"]
+        vs_plain["served_saturated"] = sf["saturated"]["voxel_updates_per_s"] / value
+        vs_plain["served_poisson_at_0.7_load"] = sf["poisson"]["voxel_updates_per_s"] / value
+    if vs_plain:
+        extra["forms_vs_plain_reconstructVolume"] = vs_plain
 
     line = {
         "metric": METRIC, "value": value, "unit": "voxel-updates/s", "n_gpus": 1,
