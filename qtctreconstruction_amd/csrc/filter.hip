@@ -24,6 +24,8 @@
 // backprojection stages from.
 #include "ct_internal.h"
 
+#include <atomic>
+
 namespace cthip {
 
 // Twiddles in stage order: the table of a butterfly span m (m = 2, 4, ..., n)
@@ -390,11 +392,11 @@ static int persistent_grid_n(int device) {
 template <int LOGN>
 static hipError_t launch_persistent_n(const FiltJob* d_jobs, const int* d_hdr, int* d_counter,
                                       int grid, hipStream_t s) {
-    static bool attr = false;
-    if (!attr) {
+    static std::atomic<bool> attr{false};  // per process: the attribute is set once per kernel
+    if (!attr.load(std::memory_order_acquire)) {
         const hipError_t e = set_lds_attr((const void*)row_filter_persistent_kernel<LOGN>);
         if (e != hipSuccess) return e;
-        attr = true;
+        attr.store(true, std::memory_order_release);
     }
     hipLaunchKernelGGL(row_filter_persistent_kernel<LOGN>, dim3(grid), dim3(256),
                        fft_lds_bytes(1 << LOGN), s, d_jobs, d_hdr, d_counter);
@@ -465,11 +467,11 @@ FiltArgs make_filt_args(const float* d_sino, int nz, int na, int bins, int sq, i
 
 template <int LOGN>
 static hipError_t launch_fft_n(const FiltArgs& p, int nzb, hipStream_t s) {
-    static bool attr_set = false;  // > 64 KB of dynamic LDS must be opted into
-    if (!attr_set) {
+    static std::atomic<bool> attr_set{false};  // > 64 KB of dynamic LDS must be opted into
+    if (!attr_set.load(std::memory_order_acquire)) {
         const hipError_t e = set_lds_attr((const void*)row_filter_fft_kernel<LOGN>);
         if (e != hipSuccess) return e;
-        attr_set = true;
+        attr_set.store(true, std::memory_order_release);
     }
     hipLaunchKernelGGL(row_filter_fft_kernel<LOGN>, dim3(p.na, nzb), dim3(256),
                        fft_lds_bytes(1 << LOGN), s, p);

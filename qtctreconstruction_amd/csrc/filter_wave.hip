@@ -34,6 +34,7 @@
 #include "ct_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <vector>
 
@@ -453,15 +454,18 @@ void wave_filter_tables(int bins, int sq, int pad, int nfft, const float* taps, 
 // row pairs and its prefetch pays.
 template <int R>
 static int wave_grid_r(int device) {
-    static int cached[64] = {};
-    if (device >= 0 && device < 64 && cached[device] > 0) return cached[device];
+    static std::atomic<int> cached[64] = {};
+    if (device >= 0 && device < 64) {
+        const int g = cached[device].load(std::memory_order_relaxed);
+        if (g > 0) return g;
+    }
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)row_filter_wave_kernel<R>,
                                                      256, 0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return 0;
     const int g = (per_cu < 1 ? 1 : per_cu) * cus;
-    if (device >= 0 && device < 64) cached[device] = g;
+    if (device >= 0 && device < 64) cached[device].store(g, std::memory_order_relaxed);
     return g;
 }
 
