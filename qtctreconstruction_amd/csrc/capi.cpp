@@ -206,6 +206,39 @@ static float2* get_wave_tables(ct_hip_ctx* c, const Geom& g, int type) {
     return d;
 }
 
+int lds_filter_length(const Geom& g) {
+    int L = 64;
+    while (L < 2 * filter_kernel_reach(g.bins, g.sq, g.pad) + 1) L <<= 1;
+    return std::min(L, g.nfft);
+}
+
+void lds_taps_host(const Geom& g, int filter, int L, float* H) {
+    std::vector<float> taps(g.nfft / 2 + 1);
+    serial_filter_taps(g.nfft, norm_filter(filter), taps.data());
+    taps[g.nfft / 2] = 0.0f;
+    if (L == g.nfft) std::copy(taps.begin(), taps.end(), H);
+    else short_filter_taps(g.bins, g.sq, g.pad, g.nfft, taps.data(), L, H);
+}
+
+// Taps of the LDS FFT form at a length L shorter than Serial's nfft (see
+// filter_kernel_reach), cached per (bins, type): bins in 23-26, 46-53 and
+// 725-848 take them, six filter types, so the cache holds at most 816.
+static float* get_short_taps(ct_hip_ctx* c, const Geom& g, int type, int L) {
+    type = norm_filter(type);
+    const auto k = std::make_pair(g.bins, type);
+    auto it = c->short_taps.find(k);
+    if (it != c->short_taps.end()) return it->second;
+    std::vector<float> H(L / 2 + 1);
+    lds_taps_host(g, type, L, H.data());
+    float* d = static_cast<float*>(dmalloc(c, sizeof(float) * H.size()));
+    hip_check(hipMemcpyAsync(d, H.data(), sizeof(float) * H.size(), hipMemcpyHostToDevice,
+                             c->stream),
+              "upload short taps");
+    hip_check(hipStreamSynchronize(c->stream), "short taps");
+    c->short_taps[k] = d;
+    return d;
+}
+
 FiltArgs filter_args(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int filter,
                      float4* filt) {
     const Geom g = geometry(bins);
@@ -218,8 +251,9 @@ FiltArgs filter_args(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bin
         return p;
     }
     check_fft(g);
-    return make_filt_args(d_sino, nz, na, bins, g.sq, g.pad, g.nfft, get_filter(c, g.nfft, filter),
-                          get_twiddles(c, g.nfft), filt);
+    const int L = lds_filter_length(g);
+    const float* H = L < g.nfft ? get_short_taps(c, g, filter, L) : get_filter(c, g.nfft, filter);
+    return make_filt_args(d_sino, nz, na, bins, g.sq, g.pad, L, H, get_twiddles(c, L), filt);
 }
 
 hipError_t launch_filter(const FiltArgs& p, hipStream_t s) {
@@ -625,9 +659,11 @@ int ct_hip_clear_workspace(ct_hip_ctx* c) {
         release_host_paths(c);
         for (auto& kv : c->twiddles) dfree(c, kv.second);
         for (auto& kv : c->wave_tabs) dfree(c, kv.second);
+        for (auto& kv : c->short_taps) dfree(c, kv.second);
         c->filters.clear();
         c->twiddles.clear();
         c->wave_tabs.clear();
+        c->short_taps.clear();
     });
 }
 

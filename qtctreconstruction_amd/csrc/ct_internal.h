@@ -163,6 +163,13 @@ constexpr int kWaveM = 512;
 constexpr int kWaveTabEntries = 1600;
 constexpr int kWaveHsFloats = 64 * 8 * 3;
 int wave_filter_r(int bins, int sq, int pad, int nfft);
+// D: the largest kernel offset |i - pad - e| the kept outputs meet. Any
+// circular length L >= 2D + 1 gives Serial's outputs with the kernel
+// g[d mod L] = h[d] (|d| <= D); short_filter_taps gives g's real spectrum
+// Hs[0 .. L/2] (FP64 from the reference's taps, rounded once), the taps of
+// the LDS FFT form at length L.
+int filter_kernel_reach(int bins, int sq, int pad);
+void short_filter_taps(int bins, int sq, int pad, int nfft, const float* taps, int L, float* H);
 void wave_filter_tables(int bins, int sq, int pad, int nfft, const float* taps, float2* tw,
                         float* hs);
 hipError_t launch_row_filter_wave(const FiltArgs& p, hipStream_t s);

@@ -138,6 +138,8 @@ struct ct_hip_ctx {
     // (kWaveTabEntries float2), then the filter (kWaveHsFloats floats); bins
     // <= kWaveM and six types bound it to 3072 entries of 19 KB
     std::map<std::pair<int, int>, float2*> wave_tabs;
+    // LDS FFT taps at a length shorter than Serial's nfft, (bins, type) -> Hs
+    std::map<std::pair<int, int>, float*> short_taps;
     struct Trig {
         float* cos = nullptr;
         float* sin = nullptr;
@@ -239,6 +241,11 @@ cthip::FiltArgs filter_args(ct_hip_ctx* c, const float* d_sino, int nz, int na, 
 // Host copies of the wave form's tables (kWaveTabEntries float2, kWaveHsFloats
 // floats); R = wave_filter_r(...) > 0 for this geometry.
 void wave_tables_host(const Geom& g, int filter, float2* tw, float* hs);
+// Transform length of the LDS FFT form: the smallest power of two (>= 64)
+// that holds the kernel's offsets, 2D + 1, or Serial's nfft if that is not
+// longer; lds_taps_host gives its L/2 + 1 taps (Serial's own at nfft).
+int lds_filter_length(const Geom& g);
+void lds_taps_host(const Geom& g, int filter, int L, float* H);
 hipError_t launch_filter(const cthip::FiltArgs& p, hipStream_t s);
 // Toeplitz tables of the GEMM row filter forms, cached per (nfft, type, bins)
 // and made on the context stream (which the call drains).

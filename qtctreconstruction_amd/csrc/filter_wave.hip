@@ -401,16 +401,60 @@ __global__ __launch_bounds__(128) void row_filter_wave_persistent_kernel(
 
 // ---------------------------------------------------------------- host side
 
+int filter_kernel_reach(int bins, int sq, int pad) {
+    return std::max(pad + bins - 1, sq - 1 - pad);
+}
+
 int wave_filter_r(int bins, int sq, int pad, int nfft) {
     if (bins < 1 || bins > kWaveM || nfft < 512) return 0;
-    const int D = std::max(pad + bins - 1, sq - 1 - pad);
+    const int D = filter_kernel_reach(bins, sq, pad);
     const int r = (2 * D + 1 + kWaveM - 1) / kWaveM;
     return r <= 3 ? r : 0;
 }
 
+namespace {
+constexpr double kPi = 3.14159265358979323846;
+
+// h[d] = IDFT_nfft(H)[d] for d = 0 .. D, in FP64, from the reference's FP32
+// taps H (real and even; the Nyquist bin taps[nfft/2] is taken as 0, as
+// Serial's inverse transform drops it).
+std::vector<double> kernel_offsets(int nfft, const float* taps, int D) {
+    std::vector<double> cn(nfft), h(D + 1);
+    for (int i = 0; i < nfft; ++i) cn[i] = std::cos(2.0 * kPi * i / nfft);
+    for (int d = 0; d <= D; ++d) {
+        double s = (double)taps[0];
+        for (int j = 1; j < nfft / 2; ++j) s += 2.0 * (double)taps[j] * cn[((long long)j * d) % nfft];
+        h[d] = s / nfft;
+    }
+    return h;
+}
+
+// Hs[k] = sum_{|d| <= D} h[d] W_L^(k d) for the real, even kernel g of
+// length L (cos table cl of L entries), in FP64.
+double short_spectrum_at(const std::vector<double>& h, const std::vector<double>& cl, long long k,
+                         int L) {
+    double s = h[0];
+    const int D = (int)h.size() - 1;
+    for (int d = 1; d <= D; ++d) s += 2.0 * h[d] * cl[(k * d) % L];
+    return s;
+}
+
+std::vector<double> cos_table(int L) {
+    std::vector<double> cl(L);
+    for (int i = 0; i < L; ++i) cl[i] = std::cos(2.0 * kPi * i / L);
+    return cl;
+}
+}  // namespace
+
+void short_filter_taps(int bins, int sq, int pad, int nfft, const float* taps, int L, float* H) {
+    const int D = filter_kernel_reach(bins, sq, pad);
+    const std::vector<double> h = kernel_offsets(nfft, taps, D);
+    const std::vector<double> cl = cos_table(L);
+    for (int k = 0; k <= L / 2; ++k) H[k] = (float)short_spectrum_at(h, cl, k, L);
+}
+
 void wave_filter_tables(int bins, int sq, int pad, int nfft, const float* taps, float2* tw,
                         float* hs) {
-    constexpr double kPi = 3.14159265358979323846;
     const int R = wave_filter_r(bins, sq, pad, nfft);
     const int L = kWaveM * R;
     auto W = [&](long long k, long long n) {
@@ -426,25 +470,16 @@ void wave_filter_tables(int bins, int sq, int pad, int nfft, const float* taps, 
     }
     for (int q = 0; q < 8; ++q)
         for (int k = 0; k < 8; ++k) tw[kWaveTw2 + 8 * k + q] = W((long long)q * k, 64);
-    // h[d] = IDFT_nfft(H)[d] for |d| <= D (H real and even, H[nfft/2] = 0),
-    // then Hs[k] = sum_{|d| <= D} h[d] W_L^(k d), all in FP64
-    const int D = std::max(pad + bins - 1, sq - 1 - pad);
-    std::vector<double> cn(nfft), cl(L), h(D + 1);
-    for (int i = 0; i < nfft; ++i) cn[i] = std::cos(2.0 * kPi * i / nfft);
-    for (int i = 0; i < L; ++i) cl[i] = std::cos(2.0 * kPi * i / L);
-    for (int d = 0; d <= D; ++d) {
-        double s = (double)taps[0];
-        for (int j = 1; j < nfft / 2; ++j) s += 2.0 * (double)taps[j] * cn[((long long)j * d) % nfft];
-        h[d] = s / nfft;
-    }
+    // the short kernel's spectrum at the points each lane holds, over L
+    const std::vector<double> h = kernel_offsets(nfft, taps, filter_kernel_reach(bins, sq, pad));
+    const std::vector<double> cl = cos_table(L);
     for (int lane = 0; lane < 64; ++lane) {
         const int m0 = lane >> 3, m1 = lane & 7;
         for (int r = 0; r < R; ++r)
             for (int j = 0; j < 8; ++j) {
                 const long long k = (long long)R * (m0 + 8 * m1 + 64 * j) + r;
-                double s = h[0];
-                for (int d = 1; d <= D; ++d) s += 2.0 * h[d] * cl[(k * d) % L];
-                hs[((2 * r + j / 4) * 64 + lane) * 4 + (j & 3)] = (float)(s / L);
+                hs[((2 * r + j / 4) * 64 + lane) * 4 + (j & 3)] =
+                    (float)(short_spectrum_at(h, cl, k, L) / L);
             }
     }
 }

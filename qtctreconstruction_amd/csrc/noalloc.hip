@@ -133,13 +133,13 @@ static void from_sinos_noalloc(ct_hip_ctx* c, const float* d_sinos, int nz, int 
         fa = make_filt_args(d_sinos, nz, na, bins, g.sq, g.pad, g.nfft, d_whs, d_wtw, d_filt);
         fa.wave_r = wave_r;
     } else {
-        serial_filter_taps(g.nfft, norm_filter(filter), taps);
-        taps[g.nfft / 2] = 0.0f;
-        hip_check(hipMemcpyAsync(d_H, taps, sizeof(float) * (g.nfft / 2 + 1),
-                                 hipMemcpyHostToDevice, c->stream),
+        const int L = lds_filter_length(g);
+        lds_taps_host(g, filter, L, taps);
+        hip_check(hipMemcpyAsync(d_H, taps, sizeof(float) * (L / 2 + 1), hipMemcpyHostToDevice,
+                                 c->stream),
                   "noalloc: filter taps");
-        hip_check(launch_twiddles(g.nfft, d_tw, c->stream), "noalloc: twiddles");
-        fa = make_filt_args(d_sinos, nz, na, bins, g.sq, g.pad, g.nfft, d_H, d_tw, d_filt);
+        hip_check(launch_twiddles(L, d_tw, c->stream), "noalloc: twiddles");
+        fa = make_filt_args(d_sinos, nz, na, bins, g.sq, g.pad, L, d_H, d_tw, d_filt);
     }
     // the tuned configuration when this (N, na) class has been tuned, else the
     // context's: every configuration gives the same bits, and tuning allocates

@@ -149,13 +149,10 @@ def test_fft_filter_every_length(backend, O, bins):
         assert O.rel_l2(got, ref) <= TOL, (bins, filt)
 
 
-@pytest.mark.parametrize("bins", [91, 150, 212, 213, 300, 424, 425, 512])
-def test_wave_filter_every_split(backend, O, bins):
-    """Rows of 91 .. 512 bins take the short-transform filter: R = 1, 2, 3
-    sub-transforms of 512 points (the first and last width of each R). The
-    filtered rows of a stack whose last z-block holds one row (the second
-    wavefront's rows and the first's odd row read as zero) match Serial's
-    filtered rows at 1e-5, row by row, and the FP64 padded convolution."""
+def _filtered_rows_parity(backend, O, bins):
+    """The filtered rows of a stack whose last z-block holds one row match
+    Serial's filtered rows at 1e-5, row by row, and the FP64 padded
+    convolution, for three filters."""
     rng = np.random.default_rng(bins)
     nz, na = 5, 7
     sinos = rng.standard_normal((nz, na, bins)).astype(np.float32)
@@ -172,6 +169,24 @@ def test_wave_filter_every_split(backend, O, bins):
                 assert O.rel_l2(got[z, a], ref[a]) <= TOL, (bins, filt, z, a)
             ref64 = sinos[z].astype(np.float64) @ T.T
             assert O.rel_l2(got[z], ref64) <= TOL, (bins, filt, z)
+
+
+@pytest.mark.parametrize("bins", [91, 150, 212, 213, 300, 424, 425, 512])
+def test_wave_filter_every_split(backend, O, bins):
+    """Rows of 91 .. 512 bins take the short-transform filter: R = 1, 2, 3
+    sub-transforms of 512 points (the first and last width of each R); the
+    stack's last z-block holds one row, so the second wavefront's rows and
+    the first's odd row read as zero."""
+    _filtered_rows_parity(backend, O, bins)
+
+
+@pytest.mark.parametrize("bins", [23, 26, 50, 724, 725, 727, 768, 848, 849])
+def test_lds_filter_short_length(backend, O, bins):
+    """Widths whose kernel offsets (2D + 1) fit a power of two below Serial's
+    nfft (23-26, 46-53, 725-848: 64, 128 and 2048 points instead of 128, 256
+    and 4096) take the LDS FFT at that length with the FP64-designed short
+    spectrum; 724 and 849 are the neighbours that keep Serial's length."""
+    _filtered_rows_parity(backend, O, bins)
 
 
 def test_filter_design_bit_exact(backend, O):
