@@ -326,16 +326,31 @@ void check_fbp(const ct_hip_ctx* c, int bins) {
 size_t nzb_of(int nz) { return (size_t)((nz + kZB - 1) / kZB); }
 
 // normalise + Radon of nz slices (w x h) into d_sino [nz][na][bins]; d_mm gets {min,max} per slice
+struct RadonWs {
+    unsigned* keys;
+    float4 *row, *col;
+    float* side;
+};
+
+static RadonWs radon_ws(ct_hip_ctx* c, int w, int h, int nz, int na, int bins) {
+    RadonWs ws;
+    ws.keys = ensure<unsigned>(c, c->keys, sizeof(unsigned) * 2 * (size_t)nz);
+    const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
+    ws.row = ensure<float4>(c, c->row, rzb * radon_zbytes(w, h));
+    ws.col = ensure<float4>(c, c->col, rzb * radon_zbytes(h, w));
+    const size_t nside = std::max(radon_side_floats(w, bins, nz, na),
+                                  radon_side_floats(h, bins, nz, na));
+    ws.side = nside ? ensure<float>(c, c->side, sizeof(float) * nside) : nullptr;
+    return ws;
+}
+
 void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na, int bins,
                  float* d_sino, float* d_mm) {
     check_radon(w, h, bins);
-    unsigned* keys = ensure<unsigned>(c, c->keys, sizeof(unsigned) * 2 * (size_t)nz);
-    const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
-    float4* row = ensure<float4>(c, c->row, rzb * radon_zbytes(w, h));
-    float4* col = ensure<float4>(c, c->col, rzb * radon_zbytes(h, w));
-    const size_t nside = std::max(radon_side_floats(w, bins, nz, na),
-                                  radon_side_floats(h, bins, nz, na));
-    float* side = nside ? ensure<float>(c, c->side, sizeof(float) * nside) : nullptr;
+    const RadonWs ws = radon_ws(c, w, h, nz, na, bins);
+    unsigned* keys = ws.keys;
+    float4 *row = ws.row, *col = ws.col;
+    float* side = ws.side;
     hip_check(hipEventRecord(c->evr[0], c->stream), "event");
     hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
@@ -419,12 +434,25 @@ int bp_config_for(ct_hip_ctx* c, int N, int na) {
 }
 
 // filter + backprojection of nz sinograms [nz][na][bins] onto N x N
+static float4* fbp_ws(ct_hip_ctx* c, int nz, int na, int bins) {
+    if (c->filter_backend == CT_FILTER_BACKEND_GEMM_FP8)
+        (void)ensure<float>(c, c->rmax, sizeof(float) * (size_t)nz * na);
+    return ensure<float4>(c, c->filt,
+                          sizeof(float4) * nzb_of(nz) * (size_t)na * geometry(bins).sq);
+}
+
+void reserve_stages(ct_hip_ctx* c, int w, int h, int nz, int na, int bins, bool with_radon,
+                    bool with_fbp) {
+    if (with_radon) (void)radon_ws(c, w, h, nz, na, bins);
+    if (with_fbp) (void)fbp_ws(c, nz, na, bins);
+}
+
 void stage_fbp(ct_hip_ctx* c, const float* d_sino, int nz, int na, int bins, int N, int filter,
                const float* d_mm, float* d_out) {
     const Geom g = geometry(bins);
     check_fbp(c, bins);
     const int cfg = bp_config_for(c, N, na);
-    float4* filt = ensure<float4>(c, c->filt, sizeof(float4) * nzb_of(nz) * (size_t)na * g.sq);
+    float4* filt = fbp_ws(c, nz, na, bins);
     hip_check(hipEventRecord(c->ev[1], c->stream), "event");
     stage_filter(c, d_sino, nz, na, bins, filter, filt);
     hip_check(hipEventRecord(c->ev[2], c->stream), "event");
@@ -445,20 +473,25 @@ void zero_timings(ct_hip_ctx* c) {
 // (ev[0] -> ev[1], its walks in evr) when with_radon, the filter (ev[1] ->
 // ev[2]) and the backprojection (ev[2] -> ev[3]) when with_fbp.
 void add_timings(ct_hip_ctx* c, bool with_radon, bool with_fbp) {
-    hip_check(hipEventSynchronize(with_fbp ? c->ev[3] : c->ev[1]), "event sync");
+    add_timings(c, c->ev, c->evr, with_radon, with_fbp);
+}
+
+void add_timings(ct_hip_ctx* c, const hipEvent_t* ev, const hipEvent_t* evr, bool with_radon,
+                 bool with_fbp) {
+    hip_check(hipEventSynchronize(with_fbp ? ev[3] : ev[1]), "event sync");
     float t01 = 0, t12 = 0, t23 = 0;
     if (with_radon) {
         ++c->radon_slabs;
-        hip_check(hipEventElapsedTime(&t01, c->ev[0], c->ev[1]), "elapsed");
+        hip_check(hipEventElapsedTime(&t01, ev[0], ev[1]), "elapsed");
         for (int k = 0; k < 4; ++k) {
             float r = 0;
-            hip_check(hipEventElapsedTime(&r, c->evr[k], c->evr[k + 1]), "elapsed");
+            hip_check(hipEventElapsedTime(&r, evr[k], evr[k + 1]), "elapsed");
             c->radon_ms[k] += r;
         }
     }
     if (with_fbp) {
-        hip_check(hipEventElapsedTime(&t12, c->ev[1], c->ev[2]), "elapsed");
-        hip_check(hipEventElapsedTime(&t23, c->ev[2], c->ev[3]), "elapsed");
+        hip_check(hipEventElapsedTime(&t12, ev[1], ev[2]), "elapsed");
+        hip_check(hipEventElapsedTime(&t23, ev[2], ev[3]), "elapsed");
     }
     c->timings[0] += t01;
     c->timings[1] += t12;
@@ -469,13 +502,13 @@ void add_timings(ct_hip_ctx* c, bool with_radon, bool with_fbp) {
 // Whole-buffer host transfers through the pinned stager, ordered after the
 // work already enqueued on the context's stream; both return when done.
 void to_device(ct_hip_ctx* c, void* d_dst, const void* h_src, size_t bytes) {
-    hip_check(hipEventRecord(c->pev[1][0], c->stream), "event");
-    stager(c).upload(d_dst, h_src, bytes, c->pev[1][0]);
+    hip_check(hipEventRecord(c->mark, c->stream), "event");
+    stager(c).upload(d_dst, h_src, bytes, c->mark);
 }
 
 void to_host(ct_hip_ctx* c, void* h_dst, const void* d_src, size_t bytes) {
-    hip_check(hipEventRecord(c->pev[1][0], c->stream), "event");
-    stager(c).download(h_dst, d_src, bytes, c->pev[1][0]);
+    hip_check(hipEventRecord(c->mark, c->stream), "event");
+    stager(c).download(h_dst, d_src, bytes, c->mark);
 }
 
 void bind_device(ct_hip_ctx* c) { hip_check(hipSetDevice(c->device), "hipSetDevice"); }
@@ -566,6 +599,20 @@ int ct_hip_device_count(void) {
     return n;
 }
 
+// The context's events that order work without timing it.
+static std::vector<hipEvent_t*> sync_events(ct_hip_ctx* c) {
+    std::vector<hipEvent_t*> v{&c->mark};
+    for (int p = 0; p < 2; ++p) {
+        v.push_back(&c->up_done[p]);
+        v.push_back(&c->comp_done[p]);
+    }
+    for (int t = 0; t < 3; ++t) {
+        v.push_back(&c->dl_done[t]);
+        for (auto& e : c->dl_piece[t]) v.push_back(&e);
+    }
+    return v;
+}
+
 int ct_hip_create(int device, size_t hbm_budget_bytes, ct_hip_ctx** out) {
     if (!out) return CT_ERR_INVALID;
     *out = nullptr;
@@ -587,9 +634,8 @@ int ct_hip_create(int device, size_t hbm_budget_bytes, ct_hip_ctx** out) {
         for (auto& e : set) ok = ok && hipEventCreate(&e) == hipSuccess;
     for (auto& set : c->evrs)
         for (auto& e : set) ok = ok && hipEventCreate(&e) == hipSuccess;
-    for (auto& set : c->pev)
-        for (auto& e : set)
-            ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
+    for (hipEvent_t* e : sync_events(c))
+        ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         ct_hip_destroy(c);
         return CT_ERR_HIP;
@@ -620,9 +666,8 @@ void ct_hip_destroy(ct_hip_ctx* c) {
     for (auto& set : c->evrs)
         for (auto& e : set)
             if (e) (void)hipEventDestroy(e);
-    for (auto& set : c->pev)
-        for (auto& e : set)
-            if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t* e : sync_events(c))
+        if (*e) (void)hipEventDestroy(*e);
     if (c->h2d) (void)hipStreamDestroy(c->h2d);
     if (c->d2h) (void)hipStreamDestroy(c->d2h);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -687,16 +732,16 @@ int ct_hip_free(ct_hip_ctx* c, void* p) {
 int ct_hip_upload(ct_hip_ctx* c, void* dst, const void* src, size_t bytes) {
     return guarded(c, [&] {
         need(dst && src, "upload: null pointer");
-        hip_check(hipEventRecord(c->pev[1][0], c->stream), "event");
-        stager(c).upload(dst, src, bytes, c->pev[1][0]);
+        hip_check(hipEventRecord(c->mark, c->stream), "event");
+        stager(c).upload(dst, src, bytes, c->mark);
     });
 }
 
 int ct_hip_download(ct_hip_ctx* c, void* dst, const void* src, size_t bytes) {
     return guarded(c, [&] {
         need(dst && src, "download: null pointer");
-        hip_check(hipEventRecord(c->pev[1][0], c->stream), "event");
-        stager(c).download(dst, src, bytes, c->pev[1][0]);
+        hip_check(hipEventRecord(c->mark, c->stream), "event");
+        stager(c).download(dst, src, bytes, c->mark);
     });
 }
 

@@ -39,7 +39,8 @@ struct BatchGraph;  // batch.cpp
 
 // Parallel memcpy between pageable caller memory and pinned staging: the
 // calling thread and `workers` pool threads each take chunks of one copy at a
-// time (hostpath.cpp). Below 4 MB the caller copies alone.
+// time (hostpath.cpp); one copy runs at a time, whichever thread asks. Below
+// 1 MB the caller copies alone.
 class CopyPool {
 public:
     explicit CopyPool(int workers);
@@ -54,15 +55,15 @@ private:
     std::vector<std::thread> th_;
     size_t nworkers_ = 0;
     std::mutex job_mu_;  // one copy at a time
-    std::mutex mu_;
-    std::condition_variable cv_, cv_done_;
+    std::mutex mu_;      // posting a copy; sleeping workers wait on cv_
+    std::condition_variable cv_;
     char* dst_ = nullptr;
     const char* src_ = nullptr;
     size_t bytes_ = 0, chunk_ = 0, nchunks_ = 0;
     std::atomic<size_t> next_{0};
-    size_t finished_ = 0;  // workers done with the current copy
-    uint64_t gen_ = 0;
-    bool stop_ = false;
+    std::atomic<size_t> finished_{0};  // workers done with the current copy
+    std::atomic<uint64_t> gen_{0};     // copies posted
+    std::atomic<bool> stop_{false};
 };
 
 // Host <-> device copies through two pinned chunks on a stream of its own:
@@ -84,14 +85,20 @@ struct Stager {
     ~Stager();
 };
 
-// Pinned slab buffers of the pipelined host-buffer paths (hostpath.cpp).
+// Pinned slab buffers of the pipelined host-buffer paths (hostpath.cpp):
+// two for inputs, three for outputs (a slab's output is still being copied
+// out while the next two compute).
 struct HostSlabs {
     char* in[2] = {nullptr, nullptr};
-    char* out[2] = {nullptr, nullptr};
-    char* sino[2] = {nullptr, nullptr};
+    char* out[3] = {nullptr, nullptr, nullptr};
+    char* sino[3] = {nullptr, nullptr, nullptr};
     size_t in_cap = 0, out_cap = 0, sino_cap = 0;
     ~HostSlabs();
 };
+
+// Most DMA pieces one slab download is split into (each with an event, so
+// the host copies a piece out while the next one crosses PCIe).
+constexpr int kSlabChunks = 8;
 
 // Largest number of volumes in one batch.
 constexpr int kMaxBatch = 256;
@@ -155,10 +162,11 @@ struct ct_hip_ctx {
     // Stage events: before the Radon stage, before the filter, before the
     // backprojection, after it; and for the Radon stage's breakdown, before
     // the min/max pass, before the normalisation, before the walk over the
-    // image, between the two walks, after the walk over the transpose. Two sets, so a pipelined slab records into one while the
-    // previous slab's are still to be read; ev / evr point at the current set.
-    hipEvent_t evs[2][4] = {};
-    hipEvent_t evrs[2][5] = {};
+    // image, between the two walks, after the walk over the transpose. Three
+    // sets, so a pipelined slab records into one while the two before it are
+    // still to be read; ev / evr point at the current set.
+    hipEvent_t evs[3][4] = {};
+    hipEvent_t evrs[3][5] = {};
     hipEvent_t* ev = evs[0];
     hipEvent_t* evr = evrs[0];
     double timings[4] = {0, 0, 0, 0};
@@ -184,11 +192,15 @@ struct ct_hip_ctx {
     cthost::BatchGraph* batch = nullptr;
     // host-buffer paths: copy streams, memcpy threads, pinned staging
     hipStream_t h2d = nullptr, d2h = nullptr;
-    hipEvent_t pev[3][2] = {};  // per slab parity: H2D done, compute done, D2H done
+    // slab pipeline events (no timing): upload done and compute done per input
+    // buffer, download done per output buffer and per piece of it
+    hipEvent_t up_done[2] = {}, comp_done[2] = {}, dl_done[3] = {};
+    hipEvent_t dl_piece[3][cthost::kSlabChunks] = {};
+    hipEvent_t mark = nullptr;  // context stream position a stager transfer waits for
     std::unique_ptr<cthost::CopyPool> copier;
     cthost::Stager stager;
     cthost::HostSlabs slabs;
-    Buf pin_vol[2], pout_vol[2], psino[2];  // device slab buffers of the pipeline
+    Buf pin_vol[2], pout_vol[3], psino[3];  // device slab buffers of the pipeline
 };
 
 namespace cthost {
@@ -218,6 +230,14 @@ void noalloc_init(int device);
 // Adds the stage times of the events in the current set to c->timings: the
 // Radon stage when with_radon, the filter and backprojection when with_fbp.
 void add_timings(ct_hip_ctx* c, bool with_radon, bool with_fbp);
+// The same for an explicit event set (ev: 4 stage events, evr: 5 Radon events).
+void add_timings(ct_hip_ctx* c, const hipEvent_t* ev, const hipEvent_t* evr, bool with_radon,
+                 bool with_fbp);
+// Sizes the workspaces of the Radon stage (with_radon) and of the filter and
+// backprojection stage (with_fbp) for nz slices, so later calls over at most
+// nz slices reallocate nothing (a reallocation waits for the context stream).
+void reserve_stages(ct_hip_ctx* c, int w, int h, int nz, int na, int bins, bool with_radon,
+                    bool with_fbp);
 void zero_timings(ct_hip_ctx* c);
 // The context's memcpy pool and stager, created on first use.
 CopyPool* copier(ct_hip_ctx* c);

@@ -132,8 +132,9 @@ def test_c2_volume_vs_serial_and_fp64(c2, O):
 
 def test_c2_host_buffers_pipelined_bit_identical(c2, O):
     """reconstructVolume and reconstructVolumeFromSinograms on host buffers run
-    the volume in z-slabs of 64 slices through pinned staging on three
-    streams; the result is the device-resident call's bit for bit."""
+    the volume in z-slabs (32, 64, 128, 128, 64, 64, 32 slices here) through
+    pinned staging on three streams; the result is the device-resident
+    call's bit for bit."""
     be = ct.HipBackend(0)
     params = ct.ReconstructionParams(ct.FilterType.SheppLogan, NA2)
     t0 = time.perf_counter()

@@ -386,6 +386,32 @@ def test_budget_forces_ragged_slabs_bit_identical(O):
     hb.ctx.close()
 
 
+def test_host_slab_ramp_bit_identical(O):
+    """The host-buffer paths at 512 x 512 x 300 run slabs of 32, 64, 64, 64,
+    40 and 36 slices (a ramp, a rest of 40 slices in the middle, the ragged
+    last slab), their sinogram counterpart slabs of the same counts; both
+    give the device-resident calls' results bit for bit."""
+    n, d, na = 512, 300, 90
+    vol = O.phantom(91, 10, n, n, d)
+    params = ct.ReconstructionParams(ct.FilterType.SheppLogan, na)
+    ctx = ct.Context(0)
+    d_vol = ctx.alloc(vol.nbytes).upload(vol)
+    d_out = ctx.alloc(vol.nbytes)
+    d_sino = ctx.alloc(4 * d * na * n)
+    ctx.reconstruct_volume_dev(d_vol, n, n, d, na, int(params.filter), d_out, d_sino)
+    ref = d_out.download(vol.shape)
+    ref_sino = d_sino.download((d, na, n))
+    be = ct.HipBackend(0)
+    out, sino = be.reconstructVolume(vol, params, want_sinograms=True)
+    assert _bits_equal(out, ref) and _bits_equal(sino, ref_sino)
+    ang = O.serial_angles(na)
+    got = be.reconstructVolumeFromSinograms(ref_sino, ang, params)
+    ctx.reconstruct_from_sinograms_dev(d_sino, d, na, n, ang, int(params.filter), d_out)
+    assert _bits_equal(got, d_out.download(vol.shape))
+    be.ctx.close()
+    ctx.close()
+
+
 def test_unsupported_width_is_an_error(backend):
     sino = np.ones((4, 3000), np.float32)
     with pytest.raises(ct.BackendError) as ei:
