@@ -48,8 +48,8 @@ FLOP_PER_SAMPLE_DEPOSIT = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows16_kernel": 2.953e+10,
-    "bp_kernel": 6.493e+09,
+    "radon_rows16_pair_kernel": 7.697e+10,
+    "bp_kernel": 6.469e+09,
 }
 
 
@@ -132,21 +132,30 @@ def cpu_baseline(vol_host: np.ndarray, na: int, ftype: int, target_s: float, run
                                  "OMP_PLACES": os.environ.get("OMP_PLACES")}}
 
 
+def walks_paired(n):
+    """Whether both Radon walks (over the image for the angles with |sin| <=
+    |cos|, over the transposed image for the others) take the 16-slice walk,
+    rows of up to 516 px, and so run as one launch (radon_rows16_pair_kernel)."""
+    return n + 4 <= 520
+
+
 def dominant_roofline(n, na, kt, launches):
     """Roofline entry of the kernel with the largest time per step.
 
-    Candidates: the Radon row walk (radon_rows16_kernel: one launch over the
-    image for the angles with |sin| <= |cos|, one over the transposed image
-    for the others) and the backprojection (bp_kernel). Both are FP32 VALU
-    work (no MFMA, no HBM bound: a few % of HBM); achieved = algorithmic FLOP
-    per step (9 per pixel-angle-slice deposit or voxel-update) / the kernel's
-    time per step from HIP events on the context stream.
+    Candidates: the Radon row walks (radon_rows16_pair_kernel: one launch
+    walking the image for the angles with |sin| <= |cos| and the transposed
+    image for the others; past 516 px, one launch per walk) and the
+    backprojection (bp_kernel). Both are FP32 VALU work (no MFMA, no HBM
+    bound: a few % of HBM); achieved = algorithmic FLOP per step (9 per
+    pixel-angle-slice deposit or voxel-update) / the kernel's time per step
+    from HIP events on the context stream.
     """
     samples = float(n) * n * n  # pixels x slices per angle
+    walk = "radon_rows16_pair_kernel" if walks_paired(n) else "radon_rows_kernel"
     cands = {
-        "radon_rows16_kernel": (kt["walk_image"] + kt["walk_transpose"], 2 * max(launches, 1),
-                              FLOP_PER_SAMPLE * samples * na,
-                              "Radon LDS row walk (image and transposed image)"),
+        walk: (kt["walk_image"] + kt["walk_transpose"],
+               max(launches, 1) * (1 if walks_paired(n) else 2), FLOP_PER_SAMPLE * samples * na,
+               "Radon LDS row walks (image and transposed image)"),
         "bp_kernel": (kt["backprojection"], 1, FLOP_PER_UPDATE * samples * na,
                       "voxel-driven backprojection"),
     }
@@ -185,17 +194,20 @@ def stage_rooflines(n, na, kt, n_row):
                 "compulsory_bytes": bytes_,
                 "hbm_frac_at_compulsory_bytes": bytes_ / (ms / 1e3) / 1e9 / PEAK_HBM_GBS}
     copies = 2.0 * (n + 4) * n * n * 4  # the two zero-padded normalised copies
-    return {
-        "minmax": hbm(4 * vox, kt["minmax"]),
-        "normalise": hbm(4 * vox + copies, kt["normalise"]),
-        "radon_walk_image": fp32(FLOP_PER_SAMPLE * vox * n_row, copies / 2 + 4.0 * n * n_row * n,
-                                 kt["walk_image"]),
-        "radon_walk_transpose": fp32(FLOP_PER_SAMPLE * vox * (na - n_row),
-                                     copies / 2 + 4.0 * n * (na - n_row) * n, kt["walk_transpose"]),
-        "row_filter": hbm(4.0 * n * na * n + 4.0 * n * na * sq, kt["row_filter"]),
-        "backprojection": fp32(FLOP_PER_UPDATE * vox * na, 4.0 * n * na * sq + 4 * vox,
-                               kt["backprojection"]),
-    }
+    out = {"minmax": hbm(4 * vox, kt["minmax"]), "normalise": hbm(4 * vox + copies, kt["normalise"])}
+    if walks_paired(n):
+        out["radon_walks"] = fp32(FLOP_PER_SAMPLE * vox * na, copies + 4.0 * n * na * n,
+                                  kt["walk_image"] + kt["walk_transpose"])
+    else:
+        out["radon_walk_image"] = fp32(FLOP_PER_SAMPLE * vox * n_row,
+                                       copies / 2 + 4.0 * n * n_row * n, kt["walk_image"])
+        out["radon_walk_transpose"] = fp32(FLOP_PER_SAMPLE * vox * (na - n_row),
+                                           copies / 2 + 4.0 * n * (na - n_row) * n,
+                                           kt["walk_transpose"])
+    out["row_filter"] = hbm(4.0 * n * na * n + 4.0 * n * na * sq, kt["row_filter"])
+    out["backprojection"] = fp32(FLOP_PER_UPDATE * vox * na, 4.0 * n * na * sq + 4 * vox,
+                                 kt["backprojection"])
+    return out
 
 
 def precision_form(ctx, d_sino, n, na, host_sinos_sample):
@@ -518,11 +530,16 @@ def n_row_angles(na: int) -> int:
     return int((np.abs(np.sin(th)) <= np.abs(np.cos(th))).sum())
 
 
-def api_form(vol_host, n, na, ftype, reps=3):
+def api_form(vol_host, n, na, ftype, reps=10, warm_s=0.6):
     """reconstructVolume on host buffers (ct_hip_reconstruct_volume): the
     volume in and out through pinned staging, z-slabs pipelined so the PCIe
     transfers overlap the compute. The output array is allocated and touched
-    once, as a caller reusing its buffers does."""
+    once, as a caller reusing its buffers does. Warm-up: calls for warm_s
+    seconds (their times are reported too). Measured at 512^3 x 720: after
+    seconds of host-only work, about the first dozen back-to-back calls
+    (0.5 s) take 26-40 ms, some kernels and copies in them up to 3.5x slower
+    than later (kernel and copy trace), and the calls after that 25.5-25.9
+    ms."""
     import ctypes
 
     import qtctreconstruction_amd as ct
@@ -534,7 +551,12 @@ def api_form(vol_host, n, na, ftype, reps=3):
         be.ctx._call("ct_hip_reconstruct_volume", be.ctx.handle, vol_host.ctypes.data_as(f32p),
                      n, n, n, na, ftype, out.ctypes.data_as(f32p), None)
 
-    call()
+    warm_ms = []
+    t_warm = time.perf_counter()
+    while len(warm_ms) < 2 or time.perf_counter() - t_warm < warm_s:
+        t0 = time.perf_counter()
+        call()
+        warm_ms.append((time.perf_counter() - t0) * 1e3)
     ms = []
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -544,9 +566,12 @@ def api_form(vol_host, n, na, ftype, reps=3):
     med = float(np.median(ms))
     res = {"ms_per_step": med, "runs_ms": ms, "voxel_updates_per_s": float(n) ** 3 * na / (med / 1e3),
            "device_stage_ms": float(t[3]), "pcie_bytes": 2.0 * vol_host.nbytes,
-           "note": "host volume in, host volume out: pinned staging, z-slabs of 64 slices on "
-                   "three streams (upload of slab k+1 and download of slab k-1 overlap the "
-                   "compute of slab k); median of runs, after one warm-up call"}
+           "note": "host volume in, host volume out: pinned staging, z-slabs of 32, 64, 128, "
+                   "128, 64, 64, 32 slices on three streams (upload of slab k+1 and download of "
+                   "slab k-1 overlap the compute of slab k; pieces of each slab cross PCIe while "
+                   "the host copies the next; a second host thread copies outputs out); median "
+                   f"of {reps} runs after {len(warm_ms)} warm-up calls ({warm_s} s)",
+           "warmup_runs_ms": [round(x, 2) for x in warm_ms]}
     be.ctx.close()
     return out, res
 
@@ -627,8 +652,8 @@ def main():
     updates = float(nvox) * na
     value = updates / (ms / 1e3)
     log(f"[bench] {ms:.2f} ms/step; stages radon {stage[0]:.2f} (min/max {kt['minmax']:.3f}, "
-        f"normalise {kt['normalise']:.3f}, walk over the image {kt['walk_image']:.2f}, over the "
-        f"transpose {kt['walk_transpose']:.2f}) filter {stage[1]:.2f} bp {stage[2]:.2f} ms")
+        f"normalise {kt['normalise']:.3f}, walks {kt['walk_image'] + kt['walk_transpose']:.2f}) "
+        f"filter {stage[1]:.2f} bp {stage[2]:.2f} ms")
     roofline = dominant_roofline(n, na, kt, walk_launches)
     stages_roof = stage_rooflines(n, na, kt, n_row_angles(na))
     log(f"[bench] roofline {roofline}")
@@ -729,7 +754,9 @@ def main():
                       "backprojection": stage[2]},
         "kernels_ms": kt,
         "sinogram_samples_per_s": float(n) * na * n / (ms / 1e3),
-        "stages_ms_radon": {"walk_image": kt["walk_image"], "walk_transpose": kt["walk_transpose"]},
+        "stages_ms_radon": ({"walks_one_launch": kt["walk_image"] + kt["walk_transpose"]}
+                            if walks_paired(n) else
+                            {"walk_image": kt["walk_image"], "walk_transpose": kt["walk_transpose"]}),
         "roofline": roofline,
         "stages_roofline": stages_roof,
         "api_ms_per_step": api["ms_per_step"] if api else None,

@@ -413,13 +413,16 @@ int ct_hip_last_timings(const ct_hip_ctx* ctx, double* ms4);
  * milliseconds, HIP events on the context's stream, summed over the call's
  * z-slabs: [0] the row walk over the image (angles with |sin| <= |cos|), [1]
  * the row walk over the transposed image (the other angles); *launches
- * (optional) gets the launch count of each walk (one per slab). No
- * counterpart in the reference: per-kernel timing for profiling and the
- * roofline report. CT_ERR_INVALID before any Radon stage has run. */
+ * (optional) gets the launch count of each walk (one per slab). When rows
+ * and columns are both at most 516 px the two walks run as one launch: [0]
+ * then holds that launch's time and [1] is zero. No counterpart in the
+ * reference: per-kernel timing for profiling and the roofline report.
+ * CT_ERR_INVALID before any Radon stage has run. */
 int ct_hip_last_radon_timings(const ct_hip_ctx* ctx, double* ms2, int* launches);
 /* Per-kernel milliseconds of the last call, summed over its slabs: [0] the
  * per-slice min/max pass, [1] the normalisation into the walks' copies, [2]
- * the walk over the image, [3] the walk over the transpose, [4] the row
+ * the walk over the image, [3] the walk over the transpose (both walks in
+ * [2] and zero here when they run as one launch, as above), [4] the row
  * filter, [5] the backprojection; the first n (<= 6) are written. No
  * counterpart in the reference (profiling and the roofline report). */
 int ct_hip_last_kernel_timings(const ct_hip_ctx* ctx, double* ms, int n);

@@ -314,7 +314,9 @@ class Context:
         return np.array(t[:], np.float64)
 
     def kernel_timings(self) -> dict:
-        """Per-kernel milliseconds of the last call (summed over its slabs)."""
+        """Per-kernel milliseconds of the last call (summed over its slabs); when
+        the two Radon walks run as one launch (rows and columns of at most
+        516 px), walk_image holds both and walk_transpose is zero."""
         t = (ctypes.c_double * 6)()
         self._call("ct_hip_last_kernel_timings", self.handle, t, 6)
         keys = ["minmax", "normalise", "walk_image", "walk_transpose", "row_filter",
@@ -323,7 +325,9 @@ class Context:
 
     def radon_timings(self):
         """(ms of the row walk over the image, ms of the row walk over the
-        transposed image, launches per walk) of the last Radon stage."""
+        transposed image, launches per walk) of the last Radon stage; both
+        walks' time in the first and zero in the second when they run as one
+        launch."""
         t = (ctypes.c_double * 2)()
         n = ctypes.c_int(0)
         self._call("ct_hip_last_radon_timings", self.handle, t, ctypes.byref(n))

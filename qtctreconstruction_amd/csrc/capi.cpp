@@ -362,6 +362,17 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     // lines (measured 24% slower at 1024^3, no gain at 512^3).
     hip_check(hipEventRecord(c->evr[2], c->stream), "event");
     const int n_row = c->cur.n_row;
+    if (radon_walks_paired(w, h)) {
+        // both walks in one launch: its time is the image walk's, the
+        // transpose's is zero
+        hip_check(launch_radon_walks(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin,
+                                     c->cur.alist, n_row, d_sino, side, c->stream),
+                  "radon walks");
+        hip_check(hipEventRecord(c->evr[3], c->stream), "event");
+        hip_check(hipEventRecord(c->evr[4], c->stream), "event");
+        c->radon_timed = true;
+        return;
+    }
     hip_check(launch_radon_rows(row, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
                                 n_row, d_sino, side, c->stream),
               "radon rows");

@@ -30,38 +30,39 @@ __host__ __device__ inline size_t radon_zbytes(int w, int h) {
 
 // XCD-grouped work order. Workgroups are dispatched to the kXcds XCDs
 // round-robin (workgroup u to XCD u % kXcds), each XCD with its own L2. A
-// launch of kXcds * slots * rounds workgroups covers nunit units (z-blocks)
-// of nslot workgroups each: in round q, units q*P .. q*P + P - 1 (P =
-// kXcds / gsz) are each walked by one group of gsz XCDs, slots = ceil(nslot
-// / gsz) workgroups per XCD, so a unit's data stays in its group's L2s. The
-// mapping is a bijection onto the (unit, slot) pairs whatever the dispatch
-// order; only the speed depends on it.
+// launch of nx * slots * rounds workgroups on nx XCDs (kXcds, or a share of
+// them: see the paired Radon walks) covers nunit units (z-blocks) of nslot
+// workgroups each: in round q, units q*P .. q*P + P - 1 (P = nx / gsz) are
+// each walked by one group of gsz XCDs, slots = ceil(nslot / gsz) workgroups
+// per XCD, so a unit's data stays in its group's L2s. The mapping is a
+// bijection onto the (unit, slot) pairs whatever the dispatch order; only the
+// speed depends on it.
 constexpr int kXcds = 8;
 __device__ __forceinline__ bool xcd_map(int u, int gsz, int slots, int nslot, int nunit,
-                                        int& unit, int& slot) {
-    const int xcd = u % kXcds, r = u / kXcds;
+                                        int& unit, int& slot, int nx = kXcds) {
+    const int xcd = u % nx, r = u / nx;
     const int round = r / slots, rs = r - round * slots;
-    unit = round * (kXcds / gsz) + xcd / gsz;
+    unit = round * (nx / gsz) + xcd / gsz;
     slot = rs * gsz + xcd % gsz;
     return unit < nunit && slot < nslot;
 }
-// XCDs per unit for nunit units of nslot workgroups. Each round takes
-// ceil(nslot / g) workgroups per XCD and there are ceil(nunit / (kXcds / g))
-// rounds; among the g within 2% of the fewest workgroups per XCD over all
+// XCDs per unit for nunit units of nslot workgroups on nx XCDs. Each round
+// takes ceil(nslot / g) workgroups per XCD and there are ceil(nunit / (nx /
+// g)) rounds; among the g within 2% of the fewest workgroups per XCD over all
 // rounds (so few units still fill every XCD), the one nearest `target` per
 // XCD and round (300 measured best for the Radon walks of 256- and 512-lane
 // blocks, about 1.5 times what an XCD holds of them at once), the smaller g
 // on a tie.
-inline int xcd_group(int nslot, int nunit, double target = 300.0) {
+inline int xcd_group(int nslot, int nunit, double target = 300.0, int nx = kXcds) {
     auto load = [&](int g) {
-        const long long P = kXcds / g;
+        const long long P = nx / g;
         return (double)((nunit + P - 1) / P) * (double)((nslot + g - 1) / g);
     };
     double best = 1e300;
-    for (int g = 1; g <= kXcds; g *= 2) best = std::min(best, load(g));
-    int pick = kXcds;
+    for (int g = 1; g <= nx; g *= 2) best = std::min(best, load(g));
+    int pick = nx;
     double dist = 1e300;
-    for (int g = 1; g <= kXcds; g *= 2) {
+    for (int g = 1; g <= nx; g *= 2) {
         if (load(g) > best * 1.02) continue;
         const double per = (double)((nslot + g - 1) / g);
         const double d = per > target ? per / target : target / per;
@@ -123,6 +124,15 @@ size_t radon_side_floats(int w, int bins, int nz, int na);
 hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, int bins,
                              const float* d_cos, const float* d_sin, const int* d_alist, int n,
                              float* d_sino, float* d_side, hipStream_t s);
+// Whether both walks of the Radon stage take the 16-slice walk (rows and
+// columns of up to 516 px); they then run as one launch.
+bool radon_walks_paired(int w, int h);
+// Both walks in one launch: the n_row angles alist[0..n_row) over the row copy
+// d_row, the others over the column copy d_col (radon_walks_paired must hold).
+hipError_t launch_radon_walks(const float4* d_row, const float4* d_col, int w, int h, int nz,
+                              int na, int bins, const float* d_cos, const float* d_sin,
+                              const int* d_alist, int n_row, float* d_sino, float* d_side,
+                              hipStream_t s);
 
 // ---- row filter ----------------------------------------------------------
 struct FiltArgs {

@@ -527,12 +527,13 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
 // B = u (the last bin's B goes nowhere and is not written).
 constexpr int kR16W = 576;
 
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
-void radon_rows16_kernel(
-    const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
+// The walk of workgroup u of its grid on nx XCDs (the body of both kernels
+// below).
+__device__ __forceinline__ void rows16_walk(
+    int u, const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
     const float* __restrict__ cos_t, const float* __restrict__ sin_t,
     const int* __restrict__ alist, float* __restrict__ sino, float* __restrict__ side,
-    int n_ang, int gsz) {
+    int n_ang, int gsz, int nx) {
     constexpr int NT = 512;
     // staged rows, one row ahead: even rows in tE, odd rows in tO (two
     // arrays, so the LDS reads of one row need not wait for the DMA into the
@@ -545,7 +546,7 @@ void radon_rows16_kernel(
     const int nap = (n_ang + 1) / 2;  // angle pairs
     const int slots = (bb * nap + gsz - 1) / gsz;
     int zp, slot;
-    if (!xcd_map(blockIdx.x, gsz, slots, bb * nap, npair, zp, slot)) return;  // whole block
+    if (!xcd_map(u, gsz, slots, bb * nap, npair, zp, slot, nx)) return;  // whole block
     const int half = __builtin_amdgcn_readfirstlane(threadIdx.x >> 9);
     const int lt = threadIdx.x & (NT - 1);
     const int ai = 2 * (slot / bb) + half;  // this half's angle, in the walk's list
@@ -669,6 +670,49 @@ void radon_rows16_kernel(
     }
 }
 
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void radon_rows16_kernel(
+    const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
+    const float* __restrict__ cos_t, const float* __restrict__ sin_t,
+    const int* __restrict__ alist, float* __restrict__ sino, float* __restrict__ side,
+    int n_ang, int gsz) {
+    rows16_walk(blockIdx.x, rowL, w, h, nz, na, bins, cos_t, sin_t, alist, sino, side, n_ang,
+                gsz, kXcds);
+}
+
+// Both walks of a Radon stage in one launch, each on half the XCDs: XCDs
+// 0-3 walk the image (a), XCDs 4-7 the transposed image (b), each in its own
+// XCD-grouped order over its four XCDs (workgroup g is on XCD g % 8; its
+// index in its walk's grid is (g / 8) * 4 + g % 4). The walks write disjoint
+// angles and take about the same time (as many angles each, to one). One
+// launch leaves no idle tail between the walks, which over few slices (a
+// round or two of workgroups) is much of the time: both walks over 32 slices
+// of 512^2 at 720 angles, 1.18 ms as two launches. Each XCD still walks one
+// pair of z-blocks at a time, so the L2s and the memory-side cache hold what
+// they held for the separate launches.
+struct Walk16 {
+    const float4* rows;
+    int w, h;
+    const float* cos_t;
+    const float* sin_t;
+    const int* alist;
+    int n_ang, gsz;
+    unsigned blocks;  // workgroups of the walk's grid
+};
+constexpr int kHalfXcds = kXcds / 2;
+
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void radon_rows16_pair_kernel(Walk16 a, Walk16 b, int nz, int na, int bins,
+                              float* __restrict__ sino, float* __restrict__ side) {
+    const unsigned g = blockIdx.x;
+    const bool second = (g / kHalfXcds) & 1u;
+    const Walk16& W = second ? b : a;
+    const int u = (int)((g / kXcds) * kHalfXcds + g % kHalfXcds);
+    if (u >= (int)W.blocks) return;  // whole block
+    rows16_walk(u, W.rows, W.w, W.h, nz, na, bins, W.cos_t, W.sin_t, W.alist, sino, side,
+                W.n_ang, W.gsz, kHalfXcds);
+}
+
 // Adds each bin block's last B to the next block's first bin (grid: seams x
 // angles of the walk, z).
 __global__ void radon_seam_kernel(float* __restrict__ sino, const float* __restrict__ side,
@@ -725,6 +769,28 @@ size_t radon_side_floats(int w, int bins, int nz, int na) {
     return (size_t)nz * na * ((bins + NT - 1) / NT - 1);
 }
 
+// Grid of the 16-slice walk over n angles on nx XCDs: units are pairs of
+// z-blocks, ~150 of its 1024-thread workgroups per XCD and round (40-600
+// measured alike at 512^3); a multiple of nx workgroups (none when n = 0).
+static Walk16 walk16(const float4* rows, int w, int h, int nz, int bins, const float* cos_t,
+                     const float* sin_t, const int* alist, int n, int nx) {
+    Walk16 W{rows, w, h, cos_t, sin_t, alist, n, 1, 0u};
+    if (n <= 0) return W;
+    const int nzb = (nz + kRZ - 1) / kRZ, bb = (bins + 511) / 512;
+    const int npair = (nzb + 1) / 2, nap = (n + 1) / 2;
+    W.gsz = xcd_group(bb * nap, npair, 150.0, nx);
+    const long long slots = (bb * nap + W.gsz - 1) / W.gsz, P = nx / W.gsz;
+    W.blocks = (unsigned)(nx * slots * ((npair + P - 1) / P));
+    return W;
+}
+
+static void launch_seams(float* d_sino, const float* d_side, const int* d_alist, int n, int nz,
+                         int na, int bins, int NT, hipStream_t s) {
+    const int nseam = (bins + NT - 1) / NT - 1;
+    hipLaunchKernelGGL(radon_seam_kernel, dim3((nseam * n + 255) / 256, nz), dim3(256), 0, s,
+                       d_sino, d_side, d_alist, n, na, bins, NT, nseam);
+}
+
 hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, int bins,
                              const float* d_cos, const float* d_sin, const int* d_alist, int n,
                              float* d_sino, float* d_side, hipStream_t s) {
@@ -741,12 +807,9 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
     const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
     const dim3 grid((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
     if (NT == 512) {
-        const int npair = (nzb + 1) / 2, nap = (n + 1) / 2;
-        const int g16 = xcd_group(bb * nap, npair, 150.0);
-        const long long s16 = (bb * nap + g16 - 1) / g16, P16 = kXcds / g16;
-        const dim3 grid16((unsigned)(kXcds * s16 * ((npair + P16 - 1) / P16)));
-        hipLaunchKernelGGL(radon_rows16_kernel, grid16, dim3(1024), 0, s, d_img, w, h, nz, na,
-                           bins, d_cos, d_sin, d_alist, d_sino, d_side, n, g16);
+        const Walk16 W = walk16(d_img, w, h, nz, bins, d_cos, d_sin, d_alist, n, kXcds);
+        hipLaunchKernelGGL(radon_rows16_kernel, dim3(W.blocks), dim3(1024), 0, s, d_img, w, h, nz,
+                           na, bins, d_cos, d_sin, d_alist, d_sino, d_side, n, W.gsz);
     } else if (NT == 768)
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 776, 768, true>), grid, dim3(768), 0,
                            s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
@@ -759,10 +822,28 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 368, 256, false>), grid, dim3(256),
                            0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
                            n, gsz);
+    if (bb > 1) launch_seams(d_sino, d_side, d_alist, n, nz, na, bins, NT, s);
+    return hipGetLastError();
+}
+
+bool radon_walks_paired(int w, int h) { return radon_lanes(w) == 512 && radon_lanes(h) == 512; }
+
+hipError_t launch_radon_walks(const float4* d_row, const float4* d_col, int w, int h, int nz,
+                              int na, int bins, const float* d_cos, const float* d_sin,
+                              const int* d_alist, int n_row, float* d_sino, float* d_side,
+                              hipStream_t s) {
+    const Walk16 a = walk16(d_row, w, h, nz, bins, d_cos, d_sin, d_alist, n_row, kHalfXcds);
+    const Walk16 b = walk16(d_col, h, w, nz, bins, d_sin, d_cos, d_alist + n_row, na - n_row,
+                            kHalfXcds);
+    const unsigned half = std::max(a.blocks, b.blocks);  // multiples of kHalfXcds
+    if (half == 0) return hipSuccess;
+    hipLaunchKernelGGL(radon_rows16_pair_kernel, dim3(2 * half), dim3(1024), 0, s, a, b, nz, na,
+                       bins, d_sino, d_side);
+    const int bb = (bins + 511) / 512;
     if (bb > 1) {
-        const int nseam = bb - 1;
-        hipLaunchKernelGGL(radon_seam_kernel, dim3((nseam * n + 255) / 256, nz), dim3(256), 0, s,
-                           d_sino, d_side, d_alist, n, na, bins, NT, nseam);
+        if (n_row > 0) launch_seams(d_sino, d_side, d_alist, n_row, nz, na, bins, 512, s);
+        if (na - n_row > 0)
+            launch_seams(d_sino, d_side, d_alist + n_row, na - n_row, nz, na, bins, 512, s);
     }
     return hipGetLastError();
 }
