@@ -104,14 +104,16 @@ static void store_tune_cache(int N, int na, int cfg) {
 }
 
 // Times every configuration on a synthetic volume of N x N, na angles, deep
-// enough for about 2048 tiles (the chip holds 256 CUs).
+// enough for 32768 tiles of 16 x 16 voxels x 4 slices (128 slices at 512^2;
+// at most 1 GiB): the two closest configurations at 512^2 x 720 are 3.5%
+// apart over 32 slices, within run-to-run noise, and 9% apart over 128.
 int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times) {
     load_tune_cache(c);
     auto it = c->tuned.find({N, na});
     if (it != c->tuned.end() && !times) return it->second;
     const Geom g = geometry(N);
     const int tiles16 = ((N + 15) / 16) * ((N + 15) / 16);
-    int nzb = std::max(1, 2048 * 4 / std::max(1, tiles16));
+    int nzb = std::max(1, 8192 * 4 / std::max(1, tiles16));
     const size_t per_zb = (size_t)na * g.sq * sizeof(float4) + sizeof(float) * kZB * (size_t)N * N;
     nzb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nzb, (size_t)(1u << 30) / per_zb));
     const int nz = nzb * kZB;
