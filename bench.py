@@ -393,7 +393,7 @@ def served_form(ctx, n_jobs, seed=5, pool_gb=96.0, rate_factor=0.7):
     upd = np.array([N * N * d * na for (N, d, na, f) in jobs], np.float64)
     # plain per-job HIP FBP on a sample of the mix -> jobs/s capacity, rate
     d_out = ctx.alloc(4 * 1024 * 1024 * 512)
-    sample = jobs[: min(40, n_jobs)]
+    sample = jobs[: min(200, n_jobs)]
     for N in edges:  # tune every class once before timing
         for na in angle_sets:
             ctx.reconstruct_from_sinograms_dev(pool[(N, na)], 8, na, N, angles[na], 0, d_out)
@@ -423,9 +423,12 @@ def served_form(ctx, n_jobs, seed=5, pool_gb=96.0, rate_factor=0.7):
     host_ups = float(sum(N * N * d * na for (N, d, na, f) in hsample)) / t_host
     log(f"[bench] serving: plain through host buffers {host_ups:.3e} upd/s "
         f"({len(hsample)} jobs, PCIe included)")
-    mean_job_s = t_plain / len(sample)
-    lam = rate_factor / mean_job_s
-    log(f"[bench] serving: plain {plain_ups:.3e} upd/s, {1/mean_job_s:.1f} jobs/s -> rate {lam:.1f}/s")
+    # plain jobs/s of this mix: the plain rate over the mix's mean job (the
+    # sample's jobs are lighter or heavier than the mix's by chance)
+    plain_jobs_s = plain_ups / float(upd.mean())
+    lam = rate_factor * plain_jobs_s
+    log(f"[bench] serving: plain {plain_ups:.3e} upd/s, {plain_jobs_s:.1f} jobs/s of the mix "
+        f"-> rate {lam:.1f}/s")
 
     poisson_arrivals = np.cumsum(rng.exponential(1.0 / lam, n_jobs))
     host_buf = np.zeros(max(N * N * d for (N, d, na, f) in jobs), np.float32)
@@ -505,7 +508,12 @@ def served_form(ctx, n_jobs, seed=5, pool_gb=96.0, rate_factor=0.7):
     log(f"[bench] serving poisson fifo: {poisson_fifo}")
     for d_s in pool.values():
         d_s.free()
+    offered = lam * float(upd.mean())
     return {"jobs": n_jobs, "rate_per_s": lam, "rate_factor_of_plain": rate_factor,
+            "mean_job_voxel_updates": float(upd.mean()),
+            "offered_load_vs_plain": offered / plain_ups,
+            "offered_load_vs_saturated_server": offered / saturated["voxel_updates_per_s"],
+            "plain_sample_jobs": len(sample),
             "plain_per_job_voxel_updates_per_s": plain_ups,
             "plain_per_job_host_buffers_voxel_updates_per_s": host_ups,
             "poisson_vs_plain_device": poisson["voxel_updates_per_s"] / plain_ups,

@@ -9,16 +9,21 @@
 //
 // The backprojection tile configuration (voxel tile, angles per LDS stage)
 // comes from the autotuner, which times every instantiated configuration on
-// a synthetic volume of the job's class and caches the winner per (N, na),
-// optionally in a file named by CT_HIP_TUNE_CACHE.
+// a synthetic volume of the job's class and caches the winner per (N, na):
+// in the context, for every context of the process on the same device (a
+// tuning takes up to a second at 1024^2 x 1440 angles, and a server's
+// context, made with the server, would otherwise tune each class again while
+// its jobs wait), and optionally in a file named by CT_HIP_TUNE_CACHE.
 #include "ctx.h"
 
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <mutex>
 #include <numeric>
 #include <sstream>
+#include <tuple>
 
 using namespace cthip;
 
@@ -85,6 +90,25 @@ void batch_release(ct_hip_ctx* c) {
     c->batch = nullptr;
 }
 
+// Tuning results of the process: (device, N, na) -> configuration. Every
+// configuration gives the same result bit for bit, so sharing them changes
+// speed only.
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int>, int> g_tuned;
+
+static void share_tuned(int device, int N, int na, int cfg) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[std::make_tuple(device, N, na)] = cfg;
+}
+
+static bool shared_tuned(int device, int N, int na, int* cfg) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple(device, N, na));
+    if (it == g_tuned.end()) return false;
+    *cfg = it->second;
+    return true;
+}
+
 static void load_tune_cache(ct_hip_ctx* c) {
     if (c->tune_cache_loaded) return;
     c->tune_cache_loaded = true;
@@ -111,6 +135,8 @@ int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times) {
     load_tune_cache(c);
     auto it = c->tuned.find({N, na});
     if (it != c->tuned.end() && !times) return it->second;
+    int shared = 0;
+    if (!times && shared_tuned(c->device, N, na, &shared)) return c->tuned[{N, na}] = shared;
     const Geom g = geometry(N);
     const int tiles16 = ((N + 15) / 16) * ((N + 15) / 16);
     int nzb = std::max(1, 8192 * 4 / std::max(1, tiles16));
@@ -181,6 +207,7 @@ int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times) {
     dfree(c, out);
     const int best = (int)(std::min_element(ms.begin(), ms.end()) - ms.begin());
     c->tuned[{N, na}] = best;
+    share_tuned(c->device, N, na, best);
     store_tune_cache(N, na, best);
     if (times) *times = ms;
     return best;

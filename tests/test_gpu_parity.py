@@ -108,6 +108,37 @@ def test_sinogram_indexing_bit_exact(backend, O, h, w, na, bins):
         assert _bits_equal(got.data, ref), (y, x)
 
 
+@pytest.mark.parametrize("h,w,na,bins", [(9, 9, 180, 1), (7, 9, 180, 2), (9, 8, 90, 3),
+                                         (24, 24, 180, 24), (29, 31, 90, 31)])
+def test_sinogram_first_bin_edge_bit_exact(backend, O, h, w, na, bins):
+    """Pixels whose s lies just either side of 0 at some angle (i0 = -1 drops
+    both deposits, i0 = 0 keeps them): with a detector of 1 to 3 bins, dc is
+    0 or 1 and s carries bits below 2^-23, so a fraction formed as s + 1
+    would round. One pixel per slice, bit for bit against Serial."""
+    ang = O.serial_angles(na)
+    cs = np.array([np.cos(O.deg2rad(a)) for a in ang], np.float32)
+    sn = np.array([np.sin(O.deg2rad(a)) for a in ang], np.float32)
+    cx, cy = np.float32(w - 1) * np.float32(0.5), np.float32(h - 1) * np.float32(0.5)
+    dc = np.float32(bins // 2)
+    xx = (np.arange(w, dtype=np.float32) - cx)[None, :]
+    yy = (np.arange(h, dtype=np.float32) - cy)[:, None]
+    picks = set()
+    for a in range(na):
+        s = ((xx * cs[a]).astype(np.float32) + (yy * sn[a]).astype(np.float32)).astype(np.float32)
+        s = (s + dc).astype(np.float32)
+        m = np.abs(s) + (s == 0) * 9  # nearest nonzero s
+        yx = np.unravel_index(np.argmin(m), m.shape)
+        if m[yx] < 0.05:
+            picks.add((int(yx[0]), int(yx[1])))
+    assert picks
+    for y, x in sorted(picks)[:10]:
+        sl = np.zeros((h, w), np.float32)
+        sl[y, x] = np.float32(1.7)
+        got = backend.computeSinogram(sl, na, bins)
+        ref, _ = O.compute_sinogram(sl, na, bins)
+        assert _bits_equal(got.data, ref), (y, x)
+
+
 def test_empty_inputs(backend):
     s = backend.computeSinogram(np.zeros((0, 0), np.float32), 10, 10)
     assert s.data.size == 0
