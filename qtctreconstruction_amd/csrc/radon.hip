@@ -306,6 +306,9 @@ __device__ __forceinline__ void chunk_add(const RayGeom& g, float xm0, float sy,
         const float wa = __builtin_amdgcn_fmed3f(fminf(fmaf(u, g.ka, g.la), fmaf(u, kBig, 1.0f)),
                                                  0.0f, 1.0f);
         const float wb = __builtin_amdgcn_fmed3f(fminf(u, fmaf(u, g.kb, g.lb)), 0.0f, 1.0f);
+        // a later candidate that lies in no lane's interval adds +0 to every
+        // sum: the wave skips it (most waves near 0 and 90 degrees)
+        if (k > 0 && __builtin_amdgcn_ballot_w64(wa != 0.0f || wb != 0.0f) == 0) continue;
         const float p[kRZ] = {v[k][0].x, v[k][0].y, v[k][0].z, v[k][0].w,
                               v[k][1].x, v[k][1].y, v[k][1].z, v[k][1].w};
 #pragma unroll
@@ -521,7 +524,9 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
 // writes slot base + l; 64 pixels per load, so rows are padded to kR16W = 576
 // slots), one row ahead: row y is summed from one array while row y + 1
 // lands in the other, so the staging latency hides behind a row of sums.
-// Each candidate's 4 b128 LDS reads are issued before its sums. Weights:
+// Each candidate's 4 b128 LDS reads are issued before its sums; a wave whose
+// intervals hold no second pixel in the row skips the second candidate
+// (within ~10 degrees of the axes most waves do). Weights:
 // u = s - i in [0, 1) exactly when floor(s) = i (one unsigned compare of u's
 // bits); A = fl(1 - u), or 1 on the last bin (SerialBackend.cpp:117-118);
 // B = u (the last bin's B goes nowhere and is not written).
@@ -596,29 +601,34 @@ __device__ __forceinline__ void rows16_walk(
         const float x0f = row_first(g, yf);
         const int d = (int)x0f;
         const float xm0 = x0f - g.cx;
-        float4 q[2][4];
+        // SerialBackend.cpp:110, the same float operations in order, for the
+        // row's two candidates x0 and x0 + 1
+        const float s0 = xm0 * g.c + sy + g.dc;
+        const float s1 = (xm0 + 1.0f) * g.c + sy + g.dc;
+        const float u0 = s0 - g.fi, u1 = s1 - g.fi;
+        const bool in0 = __float_as_uint(u0) < 0x3f800000u;
+        const bool in1 = __float_as_uint(u1) < 0x3f800000u;
+        // A row holds a second pixel of an interval only where the interval
+        // spans two pixels (|cos| < 1); a wave none of whose 64 intervals
+        // does skips the second candidate's reads and sums, which would add
+        // +0 to every sum.
+        const bool two = __builtin_amdgcn_ballot_w64(in1) != 0;
+        auto add = [&](int k, float u, bool in) {
+            float4 q[4];
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-#pragma unroll
-            for (int gq = 0; gq < 4; ++gq) q[k][gq] = t[gq][d + k];
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            // SerialBackend.cpp:110, the same float operations in order
-            const float s = (k == 0 ? xm0 : xm0 + 1.0f) * g.c + sy + g.dc;
-            const float u = s - g.fi;
-            const bool in = __float_as_uint(u) < 0x3f800000u;
+            for (int gq = 0; gq < 4; ++gq) q[gq] = t[gq][d + k];
             const float wa = in ? fmaf(u, ka, 1.0f) : 0.0f;
             const float wb = in ? u : 0.0f;
-            const float p[16] = {q[k][0].x, q[k][0].y, q[k][0].z, q[k][0].w,
-                                 q[k][1].x, q[k][1].y, q[k][1].z, q[k][1].w,
-                                 q[k][2].x, q[k][2].y, q[k][2].z, q[k][2].w,
-                                 q[k][3].x, q[k][3].y, q[k][3].z, q[k][3].w};
+            const float p[16] = {q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w,
+                                 q[2].x, q[2].y, q[2].z, q[2].w, q[3].x, q[3].y, q[3].z, q[3].w};
 #pragma unroll
             for (int z = 0; z < 16; ++z) {
                 accA[z] = fmaf(p[z], wa, accA[z]);
                 accB[z] = fmaf(p[z], wb, accB[z]);
             }
-        }
+        };
+        add(0, u0, in0);
+        if (two) add(1, u1, in1);
     };
     stage(tE, 0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
