@@ -48,7 +48,7 @@ FLOP_PER_SAMPLE_DEPOSIT = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows16_pair_kernel": 7.697e+10,
+    "radon_rows16_pair_kernel": 8.064e+10,
     "bp_kernel": 6.469e+09,
 }
 
