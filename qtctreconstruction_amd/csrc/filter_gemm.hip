@@ -340,17 +340,25 @@ __global__ __launch_bounds__(768) void toeplitz_gemm_kernel(
         for (int t = 0; t < 4; ++t) acc[s][t] = (f4v){0.f, 0.f, 0.f, 0.f};
 
     const int nch = Kp / 32;
+    // The stager waves fetch chunk kc + 1's rows from memory before chunk
+    // kc's products and convert them into the other A buffer after them, so
+    // the fetch's latency passes under the products; one barrier per chunk
+    // (the buffer written in round kc was last read in round kc - 1).
+    if (stager) {
+        float pv[9];
+        uint4 h, l;
+        load_a_row(prow, slive, vec != 0, bins, 8 * g, pv);
+        convert_a<VARIANT>(pv, rsc, h, l);
+        Ah[0][wave][c][g] = h;
+        if (VARIANT == 2) Al[0][wave][c][g] = l;
+    }
+    __syncthreads();
     for (int kc = 0; kc < nch; ++kc) {
         const int buf = kc & 1;
         const int kb = kc * 32 + 8 * g;
-        if (stager) {
-            float pv[9];
-            uint4 h, l;
-            load_a_row(prow, slive, vec != 0, bins, kb, pv);
-            convert_a<VARIANT>(pv, rsc, h, l);
-            Ah[buf][wave][c][g] = h;
-            if (VARIANT == 2) Al[buf][wave][c][g] = l;
-        }
+        float pn[9];
+        const bool next = stager && kc + 1 < nch;
+        if (next) load_a_row(prow, slive, vec != 0, bins, kb + 32, pn);
         bf16x8 bh[4], bl[4];
         long b8[4];
         if (BLDS) {
@@ -368,9 +376,6 @@ __global__ __launch_bounds__(768) void toeplitz_gemm_kernel(
         } else {
             load_b<VARIANT>(Thi, Tlo, T8, n0, c, Kp, kb, bh, bl, b8);
         }
-        // the chunk is staged; the buffer written next was last read before
-        // this barrier
-        __syncthreads();
 #pragma unroll
         for (int s = 0; s < kZB; ++s) {
             if (VARIANT == 3) {
@@ -393,6 +398,13 @@ __global__ __launch_bounds__(768) void toeplitz_gemm_kernel(
                 }
             }
         }
+        if (next) {
+            uint4 h, l;
+            convert_a<VARIANT>(pn, rsc, h, l);
+            Ah[buf ^ 1][wave][c][g] = h;
+            if (VARIANT == 2) Al[buf ^ 1][wave][c][g] = l;
+        }
+        __syncthreads();
     }
 
     // epilogue: lane holds rows 4g..4g+3 (angles) of column c of every tile
