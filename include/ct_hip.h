@@ -275,8 +275,12 @@ int ct_hip_server_submit(ct_hip_server* srv, const ct_fbp_job* job, uint64_t* ti
  * that job failed. */
 int ct_hip_server_wait(ct_hip_server* srv, double timeout_ms, uint64_t* ticket,
                        const float** d_out, double* latency_ms);
+/* Copies a finished job's output (bytes >= its size) to host memory through
+ * the server's pinned staging; downloads of different tickets may run from
+ * several threads. */
 int ct_hip_server_download(ct_hip_server* srv, uint64_t ticket, float* host_out, size_t bytes);
-/* Returns a finished job's output to the pool. */
+/* Returns a finished job's output to the pool; refused (CT_ERR_INVALID)
+ * while a download of the same ticket is in progress. */
 int ct_hip_server_release(ct_hip_server* srv, uint64_t ticket);
 /* [0] batches [1] completed [2] max jobs in flight [3] launches overlapping a
  * running batch [4] pool bytes in use [5] pool bytes [6] ms first submit ->

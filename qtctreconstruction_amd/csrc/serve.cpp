@@ -109,6 +109,7 @@ struct Job {
     double t_submit = 0, t_admit = 0, t_done = 0;
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
     int slot = -1;
+    int readers = 0;  // downloads in progress (the output may not be released meanwhile)
 };
 
 struct Slot {
@@ -433,8 +434,10 @@ int ct_hip_server_wait(ct_hip_server* s, double timeout_ms, uint64_t* ticket, co
                        double* latency_ms) {
     if (!s || !ticket) return CT_ERR_INVALID;
     std::unique_lock<std::mutex> lk(s->mu);
+    // a negative or NaN timeout polls; past about a day it is taken as a day
+    const double t_ms = timeout_ms > 0 ? std::min(timeout_ms, 8.64e7) : 0.0;
     const auto deadline =
-        std::chrono::steady_clock::now() + std::chrono::microseconds((long long)(timeout_ms * 1000));
+        std::chrono::steady_clock::now() + std::chrono::microseconds((long long)(t_ms * 1000));
     while (s->done.empty()) {
         if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && s->done.empty()) return 1;
     }
@@ -454,6 +457,10 @@ int ct_hip_server_release(ct_hip_server* s, uint64_t ticket) {
     if (it == s->jobs.end()) return CT_ERR_INVALID;
     Job* j = it->second;
     if (j->state == 1 || j->state == 0) return CT_ERR_INVALID;
+    if (j->readers > 0) {  // a download still reads the output
+        s->err = "release: ticket " + std::to_string(ticket) + " is being downloaded";
+        return CT_ERR_INVALID;
+    }
     if (j->state == 2) s->pool.release(j->out_off);
     delete j;
     s->jobs.erase(it);
@@ -465,24 +472,34 @@ int ct_hip_server_download(ct_hip_server* s, uint64_t ticket, float* host_out, s
     if (!s || !host_out) return CT_ERR_INVALID;
     const float* src = nullptr;
     size_t nb = 0;
+    Job* j = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->jobs.find(ticket);
         if (it == s->jobs.end() || it->second->state != 2) return CT_ERR_INVALID;
-        src = it->second->d.d_out;
-        nb = it->second->out_bytes;
+        j = it->second;
+        src = j->d.d_out;
+        nb = j->out_bytes;
+        if (bytes < nb) return CT_ERR_INVALID;
+        j->readers++;  // release waits for the copy (it refuses the ticket meanwhile)
     }
-    if (bytes < nb) return CT_ERR_INVALID;
     // the job's batch has completed (state 2), so its output is final
+    int rc = CT_OK;
     try {
         (void)hipSetDevice(s->ctx->device);
         s->dl->download(host_out, src, nb, nullptr);
     } catch (const StatusError& e) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->err = e.what();
-        return e.code;
+        rc = e.code;
+    } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = e.what();
+        rc = CT_ERR_INVALID;
     }
-    return CT_OK;
+    std::lock_guard<std::mutex> lk(s->mu);
+    j->readers--;
+    return rc;
 }
 
 // stats[0] batches, [1] completed jobs, [2] max jobs in flight,
