@@ -15,16 +15,27 @@
 
 namespace cthip {
 
+// Waves per SIMD that the tile's LDS lets a CU hold (a 256-thread block is
+// one wave per SIMD; 160 KiB of LDS per CU, at most 8 waves per SIMD). The
+// kernels ask the compiler for that occupancy, which bounds their VGPRs.
+template <int T, int A, int Z>
+constexpr int bp_waves_per_eu() {
+    return (160 * 1024) / (int)sizeof(BpSmem<T, A, Z>) < 8
+               ? (160 * 1024) / (int)sizeof(BpSmem<T, A, Z>)
+               : 8;
+}
+
 template <int T, int A, bool CHECK, int Z>
-__global__ __launch_bounds__(256) void bp_kernel(BpArgs p) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<T, A, Z>())))
+void bp_kernel(BpArgs p) {
     __shared__ BpSmem<T, A, Z> sm;
     bp_tile<T, A, CHECK, Z>(p, blockIdx.x * T, blockIdx.y * T, blockIdx.z, sm);
 }
 
 template <int T, int A, int Z>
-__global__ __launch_bounds__(256) void bp_persistent_kernel(const BpJob* __restrict__ jobs,
-                                                            const int* __restrict__ hdr,
-                                                            int* __restrict__ counter) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<T, A, Z>())))
+void bp_persistent_kernel(const BpJob* __restrict__ jobs, const int* __restrict__ hdr,
+                          int* __restrict__ counter) {
     __shared__ BpSmem<T, A, Z> sm;
     __shared__ int item_s;
     const int njobs = hdr[0], total = hdr[1];

@@ -12,9 +12,11 @@
 // the angle sum runs in Serial's order.
 //
 // A tile is T x T voxels (T = 16 or 32) of ZB2 z-blocks (4 slices each,
-// ZB2 = 1 or 2); its 256 threads form a 16 x 16 grid and each owns (T/16)^2
-// voxels x 4*ZB2 slices, so one voxel's detector position and fraction serve
-// 4*ZB2 slices. Each slice's sum is the same sequence of operations whatever
+// ZB2 = 1 or 2); each of its 4 waves covers one (T/2) x (T/2) quadrant, a
+// lane (T/16)^2 voxels 8 apart x 4*ZB2 slices, so one voxel's detector
+// position and fraction serve 4*ZB2 slices. Tiles and waves whose voxels all
+// lie outside the reconstruction circle skip the angle sums (12.5% of the
+// 32-voxel tiles of a 512 x 512 slice, 16% of its waves). Each slice's sum is the same sequence of operations whatever
 // T, A and ZB2, so every configuration gives bit-identical volumes.
 // Angles go in stages of A: per stage the block stages into LDS, for every
 // angle, the detector window the tile can touch (F bins from
@@ -54,14 +56,17 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zp,
     constexpr int F = bp_window(T);
     constexpr int V = T / 16;  // voxels per thread per axis
     const int tid = threadIdx.x;
-    const int tx = tid & 15, ty = tid >> 4;
+    // each wave covers one (T/2) x (T/2) quadrant of the tile, its lanes an
+    // 8 x 8 grid there, a lane's voxels 8 apart
+    const int w = tid >> 6, l = tid & 63;
+    const int ox = (T / 2) * (w & 1) + (l & 7), oy = (T / 2) * (w >> 1) + (l >> 3);
     const float R = (float)(p.N / 2);
     const float dcb = (float)(p.sq / 2);
     float xx[V], yy[V];
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-        xx[i] = (float)(bx + tx + 16 * i) - R;
-        yy[i] = (float)(by + ty + 16 * i) - R;
+        xx[i] = (float)(bx + ox + 8 * i) - R;
+        yy[i] = (float)(by + oy + 8 * i) - R;
     }
     const float cxa = (float)bx - R, cxb = (float)(bx + T - 1) - R;
     const float cya = (float)by - R, cyb = (float)(by + T - 1) - R;
@@ -85,7 +90,24 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zp,
     g_cfloat* __restrict__ cos_t = (g_cfloat*)p.cos_t;
     g_cfloat* __restrict__ sin_t = (g_cfloat*)p.sin_t;
 
-    for (int a0 = 0; a0 < p.na; a0 += A) {
+    // Voxels outside the circle are written as 0 below whatever their sums,
+    // so a tile (or a wave's share of it) with no voxel inside skips the
+    // angle sums. The circle test x^2 + y^2 > R^2 is the epilogue's; it
+    // rounds monotonically in |x| and |y|, so when the tile's voxel nearest
+    // the centre fails it, every voxel of the tile does.
+    const float nx = cxa > 0.0f ? cxa : (cxb < 0.0f ? -cxb : 0.0f);
+    const float ny = cya > 0.0f ? cya : (cyb < 0.0f ? -cyb : 0.0f);
+    const int na_run = (nx * nx + ny * ny > R * R) ? 0 : p.na;  // block-uniform
+    bool mine = false;
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int j = 0; j < V; ++j)
+            mine |= bx + ox + 8 * i < p.N && by + oy + 8 * j < p.N &&
+                    !(xx[i] * xx[i] + yy[j] * yy[j] > R * R);
+    const bool wave_on = __builtin_amdgcn_ballot_w64(mine) != 0;  // wave-uniform
+
+    for (int a0 = 0; a0 < na_run; a0 += A) {
         const int nc = min(A, p.na - a0);
         if (tid < nc) {
             const float c = cos_t[a0 + tid], s = sin_t[a0 + tid];
@@ -112,7 +134,7 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zp,
             }
         }
         __syncthreads();
-        for (int aa = 0; aa < nc; ++aa) {
+        for (int aa = 0; aa < (wave_on ? nc : 0); ++aa) {
             const float2 cs = sm.cs[aa];
             const float c = cs.x, s = cs.y;
             // byte offset of this angle's window entry for bin 0: entry
@@ -186,7 +208,7 @@ __device__ __forceinline__ void bp_tile(const BpArgs& p, int bx, int by, int zp,
         for (int i = 0; i < V; ++i)
 #pragma unroll
             for (int j = 0; j < V; ++j) {
-                const int x = bx + tx + 16 * i, y = by + ty + 16 * j;
+                const int x = bx + ox + 8 * i, y = by + oy + 8 * j;
                 if (x >= p.N || y >= p.N) continue;
                 const float xf = (float)x - R, yf = (float)y - R;
                 const bool inside = !(xf * xf + yf * yf > R * R);
