@@ -7,7 +7,10 @@ HIP events), the Shepp-Logan row filter (sq 725: the 2048-point padded
 convolution as short transforms of 3 x 512 points) and
 voxel-driven backprojection with denormalisation, for a 512^3 phantom of 24
 seeded ellipsoids (the plain-FP32 single-GPU configuration). value =
-voxel-updates/s = 512^3 * 720 / step time.
+voxel-updates/s = 512^3 * 720 / step time, the nominal count of the workload
+(the host baseline uses the same one); the backprojection's roofline counts
+only the voxels inside the reconstruction circle, 78.5% of them, the ones
+Serial computes (SerialBackend.cpp:219-223).
 
 Prints one JSON line. Timed region: K steps between two device
 synchronisations; per-kernel times come from HIP events on the context's
@@ -48,8 +51,8 @@ FLOP_PER_SAMPLE_DEPOSIT = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows16_pair_kernel": 8.064e+10,
-    "bp_kernel": 6.469e+09,
+    "radon_rows16_pair_kernel": 8.137e+10,
+    "bp_kernel": 5.738e+09,
 }
 
 
@@ -139,6 +142,15 @@ def walks_paired(n):
     return n + 4 <= 520
 
 
+def circle_voxels(n: int) -> float:
+    """Voxels of an n^3 volume inside the reconstruction circle of each
+    slice, the ones Serial backprojects: (x - R)^2 + (y - R)^2 <= R^2 with
+    R = n/2 (SerialBackend.cpp:210-223)."""
+    r = n // 2
+    d = np.arange(n, dtype=np.int64) - r
+    return float(np.count_nonzero(d[:, None] ** 2 + d[None, :] ** 2 <= r * r)) * n
+
+
 def dominant_roofline(n, na, kt, launches):
     """Roofline entry of the kernel with the largest time per step.
 
@@ -147,8 +159,9 @@ def dominant_roofline(n, na, kt, launches):
     image for the others; past 516 px, one launch per walk) and the
     backprojection (bp_kernel). Both are FP32 VALU work (no MFMA, no HBM
     bound: a few % of HBM); achieved = algorithmic FLOP per step (9 per
-    pixel-angle-slice deposit or voxel-update) / the kernel's time per step
-    from HIP events on the context stream.
+    pixel-angle-slice deposit of every pixel, or per voxel-update of the
+    voxels inside the circle) / the kernel's time per step from HIP events on
+    the context stream.
     """
     samples = float(n) * n * n  # pixels x slices per angle
     walk = "radon_rows16_pair_kernel" if walks_paired(n) else "radon_rows_kernel"
@@ -156,7 +169,7 @@ def dominant_roofline(n, na, kt, launches):
         walk: (kt["walk_image"] + kt["walk_transpose"],
                max(launches, 1) * (1 if walks_paired(n) else 2), FLOP_PER_SAMPLE * samples * na,
                "Radon LDS row walks (image and transposed image)"),
-        "bp_kernel": (kt["backprojection"], 1, FLOP_PER_UPDATE * samples * na,
+        "bp_kernel": (kt["backprojection"], 1, FLOP_PER_UPDATE * circle_voxels(n) * na,
                       "voxel-driven backprojection"),
     }
     name = max(cands, key=lambda k: cands[k][0])
@@ -166,7 +179,7 @@ def dominant_roofline(n, na, kt, launches):
            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES.get(name),
            "kernel": name, "launches_per_step": nl, "avg_launch_ms": t_ms / nl,
            "note": f"{what}; FP32 VALU work, no MFMA; "
-                   f"{'9 FLOP per pixel-angle-slice deposit' if name.startswith('radon') else '9 FLOP per voxel-update'}"
+                   f"{'9 FLOP per pixel-angle-slice deposit' if name.startswith('radon') else '9 FLOP per voxel-update inside the circle'}"
                    "; 157.3 TF is MI355X's FP32 peak, which counts packed FP32; these kernels "
                    "are built without packed FP32 ops (78.6 TF non-packed issue ceiling)"}
     if name.startswith("radon"):
@@ -205,7 +218,7 @@ def stage_rooflines(n, na, kt, n_row):
                                            copies / 2 + 4.0 * n * (na - n_row) * n,
                                            kt["walk_transpose"])
     out["row_filter"] = hbm(4.0 * n * na * n + 4.0 * n * na * sq, kt["row_filter"])
-    out["backprojection"] = fp32(FLOP_PER_UPDATE * vox * na, 4.0 * n * na * sq + 4 * vox,
+    out["backprojection"] = fp32(FLOP_PER_UPDATE * circle_voxels(n) * na, 4.0 * n * na * sq + 4 * vox,
                                  kt["backprojection"])
     return out
 
@@ -664,6 +677,10 @@ def main():
         f"filter {stage[1]:.2f} bp {stage[2]:.2f} ms")
     roofline = dominant_roofline(n, na, kt, walk_launches)
     stages_roof = stage_rooflines(n, na, kt, n_row_angles(na))
+    cfg = args.bp_config if args.bp_config >= 0 else ctx.tuned_config(n, na)
+    tile = ctx.bp_configs()[cfg]
+    bp_tiles = {"config": cfg, "voxel_tile": tile[0], "angles_per_stage": tile[1],
+                "slices_per_thread": tile[2], "chosen_by": "fixed" if args.bp_config >= 0 else "autotuner"}
     log(f"[bench] roofline {roofline}")
 
     extra = {}
@@ -761,6 +778,7 @@ def main():
         "stages_ms": {"normalise_radon": stage[0], "row_filter": stage[1],
                       "backprojection": stage[2]},
         "kernels_ms": kt,
+        "backprojection_tiles": bp_tiles,
         "sinogram_samples_per_s": float(n) * na * n / (ms / 1e3),
         "stages_ms_radon": ({"walks_one_launch": kt["walk_image"] + kt["walk_transpose"]}
                             if walks_paired(n) else

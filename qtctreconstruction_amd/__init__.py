@@ -359,6 +359,14 @@ class Context:
         self._call("ct_hip_autotune", self.handle, N, na, ctypes.byref(best), ms, n)
         return best.value, list(ms)
 
+    def tuned_config(self, N: int, na: int) -> int:
+        """The autotuner's tile configuration for N x N slices and na angles
+        (tuning on first use); reconstructions use it unless set_bp_config
+        fixed one."""
+        best = ctypes.c_int()
+        self._call("ct_hip_autotune", self.handle, N, na, ctypes.byref(best), None, 0)
+        return best.value
+
     def fbp_batch(self, jobs, flags: int = CT_BATCH_GRAPH | CT_BATCH_AUTOTUNE):
         """jobs: list of (d_sinos DeviceBuffer, d_out DeviceBuffer, angles, nz, na, bins, filter)."""
         keep = []

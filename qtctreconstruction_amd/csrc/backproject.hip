@@ -29,7 +29,16 @@ template <int T, int A, bool CHECK, int Z>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu<T, A, Z>())))
 void bp_kernel(BpArgs p) {
     __shared__ BpSmem<T, A, Z> sm;
-    bp_tile<T, A, CHECK, Z>(p, blockIdx.x * T, blockIdx.y * T, blockIdx.z, sm);
+    // The dispatcher deals workgroups to the 8 XCDs in turn. Renumbered so
+    // that XCD k takes the k-th eighth of the (z-group, tile) order, the
+    // tiles of a z-group run on one XCD and read its filtered rows through
+    // one L2 instead of eight.
+    const int gxy = gridDim.x * gridDim.y, nb = gxy * gridDim.z;
+    int w = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    if ((nb & 7) == 0) w = (w & 7) * (nb >> 3) + (w >> 3);
+    const int zp = w / gxy, t = w - zp * gxy;
+    const int by = t / gridDim.x, bx = t - by * gridDim.x;
+    bp_tile<T, A, CHECK, Z>(p, bx * T, by * T, zp, sm);
 }
 
 template <int T, int A, int Z>
