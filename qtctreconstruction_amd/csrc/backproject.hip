@@ -5,8 +5,8 @@
 //                                 groups of Z)
 //   bp_persistent_kernel<T,A,Z>   many volumes of uneven size in one launch:
 //                                 work items (job, z-block group, tile) are
-//                                 taken from an atomic counter until the list
-//                                 is exhausted
+//                                 taken from 8 atomic cursors, one per eighth
+//                                 of the list, until all are exhausted
 //
 // (T, A, Z) = (voxel tile edge, angles per LDS stage, z-blocks per tile) is a
 // tuning choice; the instantiated set is kBpConfigs and the autotuner picks
@@ -48,25 +48,38 @@ void bp_persistent_kernel(const BpJob* __restrict__ jobs, const int* __restrict_
     __shared__ BpSmem<T, A, Z> sm;
     __shared__ int item_s;
     const int njobs = hdr[0], total = hdr[1];
-    for (;;) {
-        if (threadIdx.x == 0) item_s = atomicAdd(counter, 1);
-        __syncthreads();
-        const int item = item_s;
-        __syncthreads();
-        if (item >= total) break;
-        int j = 0;
-        while (j + 1 < njobs && jobs[j + 1].item_begin <= item) ++j;
-        const BpJob& jb = jobs[j];
-        const int tx = (jb.a.N + T - 1) / T;
-        const int per = tx * tx;
-        int local = item - jb.item_begin;
-        const int zp = local / per;  // z-block group
-        local -= zp * per;
-        const int by = local / tx, bx = local - by * tx;
-        if (jb.a.check)
-            bp_tile<T, A, true, Z>(jb.a, bx * T, by * T, zp, sm);
-        else
-            bp_tile<T, A, false, Z>(jb.a, bx * T, by * T, zp, sm);
+    // counter[0..7] are cursors into 8 contiguous eighths of the item list
+    // (items run job by job and z-group by z-group, so an eighth is a run of
+    // whole z-groups apart from its two ends). Workgroups are dealt to the 8
+    // XCDs in turn: a workgroup
+    // drains its XCD's eighth first, so a z-group's filtered rows are read
+    // through one L2, then helps with the other eighths in turn. Every
+    // workgroup leaves once all eight cursors have passed their ends.
+    const int home = blockIdx.x & 7;
+    for (int k = 0; k < 8; ++k) {
+        const int part = (home + k) & 7;
+        const int lo = (int)(((long long)total * part) >> 3);
+        const int hi = (int)(((long long)total * (part + 1)) >> 3);
+        for (;;) {
+            if (threadIdx.x == 0) item_s = lo + atomicAdd(counter + part, 1);
+            __syncthreads();
+            const int item = item_s;
+            __syncthreads();
+            if (item >= hi) break;
+            int j = 0;
+            while (j + 1 < njobs && jobs[j + 1].item_begin <= item) ++j;
+            const BpJob& jb = jobs[j];
+            const int tx = (jb.a.N + T - 1) / T;
+            const int per = tx * tx;
+            int local = item - jb.item_begin;
+            const int zp = local / per;  // z-block group
+            local -= zp * per;
+            const int by = local / tx, bx = local - by * tx;
+            if (jb.a.check)
+                bp_tile<T, A, true, Z>(jb.a, bx * T, by * T, zp, sm);
+            else
+                bp_tile<T, A, false, Z>(jb.a, bx * T, by * T, zp, sm);
+        }
     }
 }
 

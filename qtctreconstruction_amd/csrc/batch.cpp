@@ -47,7 +47,7 @@ void slot_init(ct_hip_ctx* c, BatchSlot* s, bool own_stream) {
     s->ftab = static_cast<FiltJob*>(dmalloc(c, sizeof(FiltJob) * kMaxBatch));
     s->btab = static_cast<BpJob*>(dmalloc(c, sizeof(BpJob) * kMaxBatch));
     s->hdr = static_cast<int*>(dmalloc(c, sizeof(int) * (2 + 3 * kMaxFftClasses)));
-    s->counters = static_cast<int*>(dmalloc(c, sizeof(int) * (1 + kMaxFftClasses)));
+    s->counters = static_cast<int*>(dmalloc(c, sizeof(int) * (kBpCounters + kMaxFftClasses)));
     hip_check(hipHostMalloc((void**)&s->h_ftab, sizeof(FiltJob) * kMaxBatch, hipHostMallocDefault),
               "pinned table");
     hip_check(hipHostMalloc((void**)&s->h_btab, sizeof(BpJob) * kMaxBatch, hipHostMallocDefault),
@@ -333,12 +333,12 @@ void enqueue_batch(ct_hip_ctx* c, BatchSlot* s, const ct_fbp_job* jobs, int nj, 
     const int bgrid = bp_persistent_grid(cfg, c->device);
     need(bgrid > 0, "fbpBatch: no occupancy for the persistent backprojection");
     auto enqueue = [&]() {
-        hip_check(launch_zero_ints(s->counters, 1 + ci, s->stream), "zero counters");
+        hip_check(launch_zero_ints(s->counters, kBpCounters + ci, s->stream), "zero counters");
         // kernel arguments depend only on the graph key (class count, FFT
         // lengths, grids); where each class's jobs start is in the header
         for (int k = 0; k < ci; ++k)
             hip_check(launch_row_filter_persistent(s->ftab, s->hdr + 2 + 3 * k,
-                                                   s->counters + 1 + k, cls_key[k], fgrid[k],
+                                                   s->counters + kBpCounters + k, cls_key[k], fgrid[k],
                                                    s->stream),
                       "persistent row filter");
         hip_check(launch_bp_persistent(cfg, s->btab, s->hdr, s->counters, bgrid, s->stream),

@@ -243,6 +243,8 @@ inline bool bp_needs_check(int N, int sq) {
 }
 
 int bp_persistent_grid(int cfg, int device);
+// d_counter: kBpCounters ints, zero before the launch
+constexpr int kBpCounters = 8;
 hipError_t launch_bp_persistent(int cfg, const BpJob* d_jobs, const int* d_hdr, int* d_counter,
                                 int grid, hipStream_t s);
 hipError_t launch_zero_ints(int* d_p, int n, hipStream_t s);
