@@ -128,9 +128,12 @@ static void store_tune_cache(int N, int na, int cfg) {
 }
 
 // Times every configuration on a synthetic volume of N x N, na angles, deep
-// enough for 32768 tiles of 16 x 16 voxels x 4 slices (128 slices at 512^2;
-// at most 1 GiB): the two closest configurations at 512^2 x 720 are 3.5%
-// apart over 32 slices, within run-to-run noise, and 9% apart over 128.
+// enough for 131072 tiles of 16 x 16 voxels x 4 slices, within 1 GiB and
+// 6.4e10 voxel-updates (336 slices at 512^2 x 720, 40 at 1024^2 x 1440, where
+// a tuning takes about a second). Shallower volumes rank by their last, partly
+// filled wave of workgroups: over 128 slices at 512^2 x 720 the 8-slice
+// 32-voxel tiles fill 2.7 waves and time 4.6% slower than the 4-slice
+// ones, while over the full 512 slices they are 5.5% faster.
 int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times) {
     load_tune_cache(c);
     auto it = c->tuned.find({N, na});
@@ -139,9 +142,10 @@ int autotune(ct_hip_ctx* c, int N, int na, std::vector<double>* times) {
     if (!times && shared_tuned(c->device, N, na, &shared)) return c->tuned[{N, na}] = shared;
     const Geom g = geometry(N);
     const int tiles16 = ((N + 15) / 16) * ((N + 15) / 16);
-    int nzb = std::max(1, 8192 * 4 / std::max(1, tiles16));
+    int nzb = std::max(1, 8192 * 16 / std::max(1, tiles16));
     const size_t per_zb = (size_t)na * g.sq * sizeof(float4) + sizeof(float) * kZB * (size_t)N * N;
     nzb = (int)std::max<size_t>(1, std::min<size_t>((size_t)nzb, (size_t)(1u << 30) / per_zb));
+    nzb = std::max(1, std::min(nzb, (int)(6.4e10 / ((double)N * N * na * kZB))));
     const int nz = nzb * kZB;
     float4* filt = nullptr;
     float* out = nullptr;
