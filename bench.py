@@ -51,8 +51,8 @@ FLOP_PER_SAMPLE_DEPOSIT = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows16_pair_kernel": 8.137e+10,
-    "bp_kernel": 5.738e+09,
+    "radon_rows16_pair_kernel": 8.007e+10,
+    "bp_kernel": 5.629e+09,
 }
 
 
