@@ -10,7 +10,11 @@ seeded ellipsoids (the plain-FP32 single-GPU configuration). value =
 voxel-updates/s = 512^3 * 720 / step time, the nominal count of the workload
 (the host baseline uses the same one); the backprojection's roofline counts
 only the voxels inside the reconstruction circle, 78.5% of them, the ones
-Serial computes (SerialBackend.cpp:219-223).
+Serial computes (SerialBackend.cpp:219-223). The Radon walks skip the rows
+whose 16 normalised slices are all zero (Serial skips zero pixels,
+SerialBackend.cpp:104); their roofline counts the deposits of the rows
+walked (`radon_rows_walked`), and `dense_input` gives the same step on a
+volume with no zero pixel.
 
 Prints one JSON line. Timed region: K steps between two device
 synchronisations; per-kernel times come from HIP events on the context's
@@ -151,7 +155,36 @@ def circle_voxels(n: int) -> float:
     return float(np.count_nonzero(d[:, None] ** 2 + d[None, :] ** 2 <= r * r)) * n
 
 
-def dominant_roofline(n, na, kt, launches):
+def walked_deposits(vol, na: int, n_row: int) -> dict:
+    """Deposits (pixel, angle, slice) the Radon walks run on this volume.
+
+    The 16-slice walk (rows of up to 516 px) skips, per pair of 8-slice
+    z-blocks, the rows of the image (for the n_row angles walked over it)
+    and the columns (for the others, walked over the transpose) whose 16
+    normalised slices are all zero; a normalised pixel is zero where the raw
+    one is 0 (SerialBackend.cpp:104 skips it) or the slice's minimum.
+    Wider images take the 8-slice walk, which walks every row."""
+    nz, h, w = vol.shape
+    if not walks_paired(w):
+        return {"deposits": float(w) * h * nz * na, "rows_walked_frac": 1.0,
+                "columns_walked_frac": 1.0}
+    mn = vol.reshape(nz, -1).min(axis=1)
+    live = (vol != 0) & (vol != mn[:, None, None])
+    live |= np.isnan(vol)
+    npair = (nz + 15) // 16
+    pad = npair * 16 - nz
+    if pad:  # the last pair's second z-block stands in for a missing one
+        live = np.concatenate([live, live[-8:][:pad]], axis=0)
+    g = live.reshape(npair, 16, h, w).any(axis=1)
+    rows = g.any(axis=2).sum(axis=1)  # per pair
+    cols = g.any(axis=1).sum(axis=1)
+    dep = 16.0 * (float(rows.sum()) * w * n_row + float(cols.sum()) * h * (na - n_row))
+    return {"deposits": dep, "rows_walked_frac": float(rows.sum()) / (npair * h),
+            "columns_walked_frac": float(cols.sum()) / (npair * w),
+            "nonzero_pixel_frac": float(live[:nz].mean())}
+
+
+def dominant_roofline(n, na, kt, launches, radon_dep):
     """Roofline entry of the kernel with the largest time per step.
 
     Candidates: the Radon row walks (radon_rows16_pair_kernel: one launch
@@ -159,15 +192,14 @@ def dominant_roofline(n, na, kt, launches):
     image for the others; past 516 px, one launch per walk) and the
     backprojection (bp_kernel). Both are FP32 VALU work (no MFMA, no HBM
     bound: a few % of HBM); achieved = algorithmic FLOP per step (9 per
-    pixel-angle-slice deposit of every pixel, or per voxel-update of the
-    voxels inside the circle) / the kernel's time per step from HIP events on
-    the context stream.
+    pixel-angle-slice deposit the walks run, radon_dep of them, or per
+    voxel-update of the voxels inside the circle) / the kernel's time per
+    step from HIP events on the context stream.
     """
-    samples = float(n) * n * n  # pixels x slices per angle
     walk = "radon_rows16_pair_kernel" if walks_paired(n) else "radon_rows_kernel"
     cands = {
         walk: (kt["walk_image"] + kt["walk_transpose"],
-               max(launches, 1) * (1 if walks_paired(n) else 2), FLOP_PER_SAMPLE * samples * na,
+               max(launches, 1) * (1 if walks_paired(n) else 2), FLOP_PER_SAMPLE * radon_dep,
                "Radon LDS row walks (image and transposed image)"),
         "bp_kernel": (kt["backprojection"], 1, FLOP_PER_UPDATE * circle_voxels(n) * na,
                       "voxel-driven backprojection"),
@@ -179,17 +211,17 @@ def dominant_roofline(n, na, kt, launches):
            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES.get(name),
            "kernel": name, "launches_per_step": nl, "avg_launch_ms": t_ms / nl,
            "note": f"{what}; FP32 VALU work, no MFMA; "
-                   f"{'9 FLOP per pixel-angle-slice deposit' if name.startswith('radon') else '9 FLOP per voxel-update inside the circle'}"
+                   f"{'9 FLOP per pixel-angle-slice deposit of the rows walked (rows whose 16 slices are all zero are skipped)' if name.startswith('radon') else '9 FLOP per voxel-update inside the circle'}"
                    "; 157.3 TF is MI355X's FP32 peak, which counts packed FP32; these kernels "
                    "are built without packed FP32 ops (78.6 TF non-packed issue ceiling)"}
     if name.startswith("radon"):
-        dep = FLOP_PER_SAMPLE_DEPOSIT * samples * na / (t_ms / 1e3) / 1e12
+        dep = FLOP_PER_SAMPLE_DEPOSIT * radon_dep / (t_ms / 1e3) / 1e12
         out["deposit_arithmetic_only"] = {"flop_per_sample": FLOP_PER_SAMPLE_DEPOSIT,
                                           "achieved": dep, "frac": dep / PEAK_FP32_TFLOPS}
     return out
 
 
-def stage_rooflines(n, na, kt, n_row):
+def stage_rooflines(n, na, kt, n_row, radon_dep):
     """Every stage of a step against the roofline that binds it: the streaming
     stages (min/max, normalisation, row filter) against HBM with their
     compulsory bytes, the Radon walks and the backprojection against FP32
@@ -209,7 +241,7 @@ def stage_rooflines(n, na, kt, n_row):
     copies = 2.0 * (n + 4) * n * n * 4  # the two zero-padded normalised copies
     out = {"minmax": hbm(4 * vox, kt["minmax"]), "normalise": hbm(4 * vox + copies, kt["normalise"])}
     if walks_paired(n):
-        out["radon_walks"] = fp32(FLOP_PER_SAMPLE * vox * na, copies + 4.0 * n * na * n,
+        out["radon_walks"] = fp32(FLOP_PER_SAMPLE * radon_dep, copies + 4.0 * n * na * n,
                                   kt["walk_image"] + kt["walk_transpose"])
     else:
         out["radon_walk_image"] = fp32(FLOP_PER_SAMPLE * vox * n_row,
@@ -675,8 +707,10 @@ def main():
     log(f"[bench] {ms:.2f} ms/step; stages radon {stage[0]:.2f} (min/max {kt['minmax']:.3f}, "
         f"normalise {kt['normalise']:.3f}, walks {kt['walk_image'] + kt['walk_transpose']:.2f}) "
         f"filter {stage[1]:.2f} bp {stage[2]:.2f} ms")
-    roofline = dominant_roofline(n, na, kt, walk_launches)
-    stages_roof = stage_rooflines(n, na, kt, n_row_angles(na))
+    vol_host = d_vol.download((n, n, n))
+    walked = walked_deposits(vol_host, na, n_row_angles(na))
+    roofline = dominant_roofline(n, na, kt, walk_launches, walked["deposits"])
+    stages_roof = stage_rooflines(n, na, kt, n_row_angles(na), walked["deposits"])
     cfg = args.bp_config if args.bp_config >= 0 else ctx.tuned_config(n, na)
     tile = ctx.bp_configs()[cfg]
     bp_tiles = {"config": cfg, "voxel_tile": tile[0], "angles_per_stage": tile[1],
@@ -684,9 +718,32 @@ def main():
     log(f"[bench] roofline {roofline}")
 
     extra = {}
-    vol_host = None
-    if not args.no_parity or not args.no_api:
-        vol_host = d_vol.download((n, n, n))
+    # The same step on a volume with no zero pixel (the phantom plus
+    # 1 + 0.001 x, so every slice keeps a span): the Radon walks skip no row,
+    # which is their cost on a dense volume.
+    dense = vol_host + (np.float32(1.0) + np.float32(1e-3) * np.arange(n, dtype=np.float32))
+    d_dense = ctx.alloc(4 * nvox).upload(dense)
+    dense_walk = walked_deposits(dense, na, n_row_angles(na))
+    del dense
+    ctx.reconstruct_volume_dev(d_dense, n, n, n, na, ftype, d_out)
+    ctx.synchronize()
+    kd = {}
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.reconstruct_volume_dev(d_dense, n, n, n, na, ftype, d_out)
+        for key, v in ctx.kernel_timings().items():
+            kd[key] = kd.get(key, 0.0) + v / 3
+    ctx.synchronize()
+    ms_dense = (time.perf_counter() - t0) * 1e3 / 3
+    d_dense.free()
+    extra["dense_input"] = {
+        "ms_per_step": ms_dense, "voxel_updates_per_s": updates / (ms_dense / 1e3),
+        "radon_walks_ms": kd["walk_image"] + kd["walk_transpose"], "kernels_ms": kd,
+        "rows_walked_frac": dense_walk["rows_walked_frac"],
+        "columns_walked_frac": dense_walk["columns_walked_frac"],
+        "note": "the same step on the phantom plus 1 + 0.001 x (no zero pixel): no row of the "
+                "Radon walks is skipped"}
+    log(f"[bench] dense input: {ms_dense:.2f} ms/step, walks {extra['dense_input']['radon_walks_ms']:.2f} ms")
     d_sino = None
     if not args.no_parity or "precision" in args.forms:
         d_sino = ctx.alloc(4 * n * na * n)
@@ -780,6 +837,7 @@ def main():
         "kernels_ms": kt,
         "backprojection_tiles": bp_tiles,
         "sinogram_samples_per_s": float(n) * na * n / (ms / 1e3),
+        "radon_rows_walked": walked,
         "stages_ms_radon": ({"walks_one_launch": kt["walk_image"] + kt["walk_transpose"]}
                             if walks_paired(n) else
                             {"walk_image": kt["walk_image"], "walk_transpose": kt["walk_transpose"]}),

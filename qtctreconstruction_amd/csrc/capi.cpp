@@ -329,6 +329,7 @@ size_t nzb_of(int nz) { return (size_t)((nz + kZB - 1) / kZB); }
 struct RadonWs {
     unsigned* keys;
     float4 *row, *col;
+    int* rflag;  // [zb][h] row flags of the row copy, then [zb][w] of the column copy
     float* side;
 };
 
@@ -338,6 +339,7 @@ static RadonWs radon_ws(ct_hip_ctx* c, int w, int h, int nz, int na, int bins) {
     const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
     ws.row = ensure<float4>(c, c->row, rzb * radon_zbytes(w, h));
     ws.col = ensure<float4>(c, c->col, rzb * radon_zbytes(h, w));
+    ws.rflag = ensure<int>(c, c->rflag, sizeof(int) * rzb * ((size_t)h + w));
     const size_t nside = std::max(radon_side_floats(w, bins, nz, na),
                                   radon_side_floats(h, bins, nz, na));
     ws.side = nside ? ensure<float>(c, c->side, sizeof(float) * nside) : nullptr;
@@ -355,8 +357,13 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     hip_check(launch_minmax_init(keys, nz, c->stream), "minmax init");
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
+    const int nzb = (nz + kRZ - 1) / kRZ;
+    int* rflag = ws.rflag;                     // rows of the row copy
+    int* cflag = ws.rflag + (size_t)nzb * h;  // rows of the column copy (image columns)
+    hip_check(launch_zero_ints(rflag, nzb * (h + w), c->stream), "zero row flags");
     hip_check(hipEventRecord(c->evr[1], c->stream), "event");
-    hip_check(launch_normalise(d_vol, w, h, nz, d_mm, row, col, c->stream), "normalise");
+    hip_check(launch_normalise(d_vol, w, h, nz, d_mm, row, col, rflag, cflag, c->stream),
+              "normalise");
     // The two walks write disjoint angles of the sinogram. They run one after
     // the other: side by side on two streams they evict each other's cache
     // lines (measured 24% slower at 1024^3, no gain at 512^3).
@@ -365,19 +372,19 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     if (radon_walks_paired(w, h)) {
         // both walks in one launch: its time is the image walk's, the
         // transpose's is zero
-        hip_check(launch_radon_walks(row, col, w, h, nz, na, bins, c->cur.cos, c->cur.sin,
-                                     c->cur.alist, n_row, d_sino, side, c->stream),
+        hip_check(launch_radon_walks(row, col, rflag, cflag, w, h, nz, na, bins, c->cur.cos,
+                                     c->cur.sin, c->cur.alist, n_row, d_sino, side, c->stream),
                   "radon walks");
         hip_check(hipEventRecord(c->evr[3], c->stream), "event");
         hip_check(hipEventRecord(c->evr[4], c->stream), "event");
         c->radon_timed = true;
         return;
     }
-    hip_check(launch_radon_rows(row, w, h, nz, na, bins, c->cur.cos, c->cur.sin, c->cur.alist,
-                                n_row, d_sino, side, c->stream),
+    hip_check(launch_radon_rows(row, rflag, w, h, nz, na, bins, c->cur.cos, c->cur.sin,
+                                c->cur.alist, n_row, d_sino, side, c->stream),
               "radon rows");
     hip_check(hipEventRecord(c->evr[3], c->stream), "event");
-    hip_check(launch_radon_rows(col, h, w, nz, na, bins, c->cur.sin, c->cur.cos,
+    hip_check(launch_radon_rows(col, cflag, h, w, nz, na, bins, c->cur.sin, c->cur.cos,
                                 c->cur.alist + n_row, na - n_row, d_sino, side, c->stream),
               "radon rows of the transpose");
     hip_check(hipEventRecord(c->evr[4], c->stream), "event");
@@ -692,7 +699,7 @@ size_t ct_hip_bytes_in_use(const ct_hip_ctx* c) { return c ? c->in_use : 0; }
 int ct_hip_clear_workspace(ct_hip_ctx* c) {
     return guarded(c, [&] {
         hip_check(hipStreamSynchronize(c->stream), "sync");
-        for (Buf* b : {&c->vol, &c->row, &c->col, &c->side, &c->sino, &c->filt, &c->out,
+        for (Buf* b : {&c->vol, &c->row, &c->col, &c->rflag, &c->side, &c->sino, &c->filt, &c->out,
                        &c->keys, &c->mm, &c->ell, &c->pc_cnt, &c->pc_pts, &c->pc_coord, &c->met})
             release(c, *b);
         for (auto& kv : c->trig) {

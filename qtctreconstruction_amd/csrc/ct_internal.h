@@ -113,26 +113,32 @@ hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hip
 // w = (raw == 0) ? 0 : (raw - min) * (1/span); z-blocked row copy
 // [zb][h][w + kPadX] and column copy (the transpose) [zb][w][h + kPadX],
 // kRZ = 8 slices per pixel (2 float4), kPadX zero pixels past each row.
+// d_rflag [zb][h] and d_cflag [zb][w] (zeroed beforehand) get 1 where the
+// z-block's row (column) holds a nonzero normalised pixel.
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
-                            float4* d_row, float4* d_col, hipStream_t s);
+                            float4* d_row, float4* d_col, int* d_rflag, int* d_cflag,
+                            hipStream_t s);
 // Radon of the n angles alist[0..n) over an image copy [zb][h][w + kPadX]
 // walked by rows, |sin| <= |cos| for every angle: the LDS-staged row walk.
 // Steeper angles run it on the column copy with (w, h) and (d_cos, d_sin)
 // exchanged. d_side holds radon_side_floats(w, bins, nz, na) floats (the
 // sums that cross bin blocks; none when every angle fits one block).
+// d_flag [zb][h]: the copy's row flags (launch_normalise); the 16-slice walk
+// skips the rows that are zero in both of its z-blocks.
 size_t radon_side_floats(int w, int bins, int nz, int na);
-hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, int bins,
-                             const float* d_cos, const float* d_sin, const int* d_alist, int n,
-                             float* d_sino, float* d_side, hipStream_t s);
+hipError_t launch_radon_rows(const float4* d_img, const int* d_flag, int w, int h, int nz,
+                             int na, int bins, const float* d_cos, const float* d_sin,
+                             const int* d_alist, int n, float* d_sino, float* d_side,
+                             hipStream_t s);
 // Whether both walks of the Radon stage take the 16-slice walk (rows and
 // columns of up to 516 px); they then run as one launch.
 bool radon_walks_paired(int w, int h);
 // Both walks in one launch: the n_row angles alist[0..n_row) over the row copy
 // d_row, the others over the column copy d_col (radon_walks_paired must hold).
-hipError_t launch_radon_walks(const float4* d_row, const float4* d_col, int w, int h, int nz,
-                              int na, int bins, const float* d_cos, const float* d_sin,
-                              const int* d_alist, int n_row, float* d_sino, float* d_side,
-                              hipStream_t s);
+hipError_t launch_radon_walks(const float4* d_row, const float4* d_col, const int* d_rflag,
+                              const int* d_cflag, int w, int h, int nz, int na, int bins,
+                              const float* d_cos, const float* d_sin, const int* d_alist,
+                              int n_row, float* d_sino, float* d_side, hipStream_t s);
 
 // ---- row filter ----------------------------------------------------------
 struct FiltArgs {

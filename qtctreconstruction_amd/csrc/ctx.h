@@ -137,7 +137,7 @@ struct ct_hip_ctx {
     std::string err;
     std::unordered_map<void*, size_t> sizes;  // every live device allocation
     // workspaces (grow-only, reused across calls)
-    Buf vol, row, col, side, sino, filt, out, keys, mm, ell, pc_cnt, pc_pts, pc_coord, met;
+    Buf vol, row, col, rflag, side, sino, filt, out, keys, mm, ell, pc_cnt, pc_pts, pc_coord, met;
     // cached device state
     std::map<std::pair<int, int>, float*> filters;  // (nfft, type) -> H[nfft/2+1]
     std::map<int, float2*> twiddles;                // nfft -> tw[nfft-1], stage order

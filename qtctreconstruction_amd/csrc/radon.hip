@@ -133,14 +133,24 @@ __global__ void minmax_decode_kernel(const unsigned* keys, int nz, float* mm) {
 // on consecutive 16-byte units: 1 KB contiguous per wave store in the row
 // copy, a column's 512 bytes per 32 lanes in the column copy. Tile rows are padded by one
 // float4, so the column reads (16 rows apart) spread over the banks.
+// rflag[zb][y] and cflag[zb][x] (zero beforehand) are set to 1 where row y
+// (column x) of the z-block holds a nonzero normalised pixel: a row of zeros
+// adds nothing to any bin (SerialBackend.cpp:104 skips zero pixels), and the
+// 16-slice walk skips it.
 constexpr int kNormTW = 64, kNormTH = 16;
 constexpr int kNormRow = kNormTW * 2 + 1;  // float4 per padded tile row
 __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict__ vol, int w,
                                                         int h, int nz,
                                                         const float* __restrict__ mm,
                                                         int vec4, float4* __restrict__ rowL,
-                                                        float4* __restrict__ colL) {
+                                                        float4* __restrict__ colL,
+                                                        int* __restrict__ rflag,
+                                                        int* __restrict__ cflag) {
     __shared__ float4 tile[kNormTH * kNormRow];
+    __shared__ int rowf[kNormTH], colf[kNormTW];
+    if (threadIdx.x < kNormTH) rowf[threadIdx.x] = 0;
+    if (threadIdx.x < kNormTW) colf[threadIdx.x] = 0;
+    __syncthreads();
     const int zb = blockIdx.z;
     const int bx = blockIdx.x * kNormTW, by = blockIdx.y * kNormTH;
     const int wp = w + kPadX, hp = h + kPadX;
@@ -194,6 +204,16 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
                 const float raw = v[k][j];
                 v[k][j] = (raw == 0.0f) ? 0.0f : (raw - mn[k]) * inv[k];
             }
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            bool col = false;
+#pragma unroll
+            for (int k = 0; k < kRZ; ++k) col |= v[k][j] != 0.0f;  // NaN counts
+            if (col) colf[4 * tx + j] = 1;
+            any |= col;
+        }
+        if (any) rowf[ty] = 1;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             float4* d = tile + ty * kNormRow + (4 * tx + j) * 2;
@@ -222,6 +242,13 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
         const int x = bx + c, y = by + (q >> 1);
         if (x < w && y < hp)
             c0[((size_t)x * hp + y) * 2 + (q & 1)] = tile[(q >> 1) * kNormRow + c * 2 + (q & 1)];
+    }
+    if (t < kNormTH) {
+        const int y = by + t;
+        if (y < h && rowf[t]) rflag[(size_t)zb * h + y] = 1;
+    } else if (t < kNormTH + kNormTW) {
+        const int c = t - kNormTH, x = bx + c;
+        if (x < w && colf[c]) cflag[(size_t)zb * w + x] = 1;
     }
 }
 
@@ -530,21 +557,28 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
 // u = s - i in [0, 1) exactly when floor(s) = i (one unsigned compare of u's
 // bits); A = fl(1 - u), or 1 on the last bin (SerialBackend.cpp:117-118);
 // B = u (the last bin's B goes nowhere and is not written).
+// Rows whose 16 slices are all zero add +0 to every sum and are skipped: the
+// block first lists, in LDS, the rows that its two z-blocks' row flags mark
+// (launch_normalise), then stages and sums those only, in ascending order, so
+// every sum is the one the full walk forms.
 constexpr int kR16W = 576;
 
 // The walk of workgroup u of its grid on nx XCDs (the body of both kernels
 // below).
 __device__ __forceinline__ void rows16_walk(
-    int u, const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
+    int u, const float4* __restrict__ rowL, const int* __restrict__ flag, int w, int h, int nz,
+    int na, int bins,
     const float* __restrict__ cos_t, const float* __restrict__ sin_t,
     const int* __restrict__ alist, float* __restrict__ sino, float* __restrict__ side,
     int n_ang, int gsz, int nx) {
     constexpr int NT = 512;
-    // staged rows, one row ahead: even rows in tE, odd rows in tO (two
-    // arrays, so the LDS reads of one row need not wait for the DMA into the
-    // other); [slice group (4 slices)][pixel]
+    // staged rows, one row ahead: the even entries of the row list in tE,
+    // the odd ones in tO (two arrays, so the LDS reads of one row need not
+    // wait for the DMA into the other); [slice group (4 slices)][pixel]
     __shared__ float4 tE[4][kR16W];
     __shared__ float4 tO[4][kR16W];
+    __shared__ unsigned short rows[kRadonMaxEdge];  // the rows to walk
+    __shared__ int nrows;
     const int bb = (bins + NT - 1) / NT;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int npair = (nzb + 1) / 2;
@@ -592,6 +626,22 @@ __device__ __forceinline__ void rows16_walk(
                                              16, 0, 0);
         }
     };
+    // the rows marked in either z-block (one wave, 64 rows a ballot)
+    if (wave == 0) {
+        const int* f0 = flag + (size_t)(2 * zp) * h;
+        const int* f1 = flag + (size_t)min(2 * zp + 1, nzb - 1) * h;
+        int n = 0;
+        for (int y0 = 0; y0 < h; y0 += 64) {
+            const int y = y0 + lane;
+            const bool mark = y < h && (f0[y] | f1[y]) != 0;
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(mark);
+            if (mark) rows[n + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (unsigned short)y;
+            n += __builtin_popcountll(m);
+        }
+        if (lane == 0) nrows = n;
+    }
+    __syncthreads();
+    const int nr = __builtin_amdgcn_readfirstlane(nrows);
     float accA[16], accB[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) accA[k] = accB[k] = 0.0f;
@@ -630,22 +680,35 @@ __device__ __forceinline__ void rows16_walk(
         add(0, u0, in0);
         if (two) add(1, u1, in1);
     };
-    stage(tE, 0);
+    // the list 64 entries at a time, entry k0 + l in lane l; a row is read
+    // out with v_readlane (a wave-uniform value, no LDS wait per row)
+    int k0 = 0;
+    int chunk = lane < nr ? (int)rows[lane] : 0;
+    auto listed = [&](int k) { return k < nr ? __builtin_amdgcn_readlane(chunk, k - k0) : 0; };
+    int y0 = listed(0), y1 = listed(1);
+    if (nr > 0) stage(tE, y0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
-    for (int y = 0; y < h; y += 2) {
-        // row y from tE while row y + 1 lands in tO (read last by row y - 1,
-        // before the previous barrier)
-        if (y + 1 < h) stage(tO, y + 1);
-        row(tE, y);
+    for (int k = 0; k < nr; k += 2) {
+        // listed row k from tE while row k + 1 lands in tO (read last by row
+        // k - 1, before the previous barrier)
+        if (k + 2 >= k0 + 64) {  // k + 2 and k + 3 (k even) lie in the next 64
+            k0 += 64;
+            chunk = k0 + lane < nr ? (int)rows[k0 + lane] : 0;
+        }
+        const int y2 = listed(k + 2), y3 = listed(k + 3);
+        if (k + 1 < nr) stage(tO, y1);
+        row(tE, y0);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
-        if (y + 1 < h) {
-            if (y + 2 < h) stage(tE, y + 2);
-            row(tO, y + 1);
+        if (k + 1 < nr) {
+            if (k + 2 < nr) stage(tE, y2);
+            row(tO, y1);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
         }
+        y0 = y2;
+        y1 = y3;
     }
     // bin i = A_i + B_(i-1), as in radon_rows_kernel, per half (the staging
     // arrays are free: every wave passed the last barrier)
@@ -682,12 +745,12 @@ __device__ __forceinline__ void rows16_walk(
 
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8, 8)))
 void radon_rows16_kernel(
-    const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
-    const float* __restrict__ cos_t, const float* __restrict__ sin_t,
+    const float4* __restrict__ rowL, const int* __restrict__ flag, int w, int h, int nz, int na,
+    int bins, const float* __restrict__ cos_t, const float* __restrict__ sin_t,
     const int* __restrict__ alist, float* __restrict__ sino, float* __restrict__ side,
     int n_ang, int gsz) {
-    rows16_walk(blockIdx.x, rowL, w, h, nz, na, bins, cos_t, sin_t, alist, sino, side, n_ang,
-                gsz, kXcds);
+    rows16_walk(blockIdx.x, rowL, flag, w, h, nz, na, bins, cos_t, sin_t, alist, sino, side,
+                n_ang, gsz, kXcds);
 }
 
 // Both walks of a Radon stage in one launch, each on half the XCDs: XCDs
@@ -702,6 +765,7 @@ void radon_rows16_kernel(
 // they held for the separate launches.
 struct Walk16 {
     const float4* rows;
+    const int* flag;  // [zb][h] row flags of the copy
     int w, h;
     const float* cos_t;
     const float* sin_t;
@@ -719,8 +783,8 @@ void radon_rows16_pair_kernel(Walk16 a, Walk16 b, int nz, int na, int bins,
     const Walk16& W = second ? b : a;
     const int u = (int)((g / kXcds) * kHalfXcds + g % kHalfXcds);
     if (u >= (int)W.blocks) return;  // whole block
-    rows16_walk(u, W.rows, W.w, W.h, nz, na, bins, W.cos_t, W.sin_t, W.alist, sino, side,
-                W.n_ang, W.gsz, kHalfXcds);
+    rows16_walk(u, W.rows, W.flag, W.w, W.h, nz, na, bins, W.cos_t, W.sin_t, W.alist, sino,
+                side, W.n_ang, W.gsz, kHalfXcds);
 }
 
 // Adds each bin block's last B to the next block's first bin (grid: seams x
@@ -758,12 +822,13 @@ hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hip
 }
 
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
-                            float4* d_row, float4* d_col, hipStream_t s) {
+                            float4* d_row, float4* d_col, int* d_rflag, int* d_cflag,
+                            hipStream_t s) {
     const int nzb = (nz + kRZ - 1) / kRZ;
     dim3 grid((w + kPadX + kNormTW - 1) / kNormTW, (h + kPadX + kNormTH - 1) / kNormTH, nzb);
     const int vec4 = (w % 4 == 0) && ((uintptr_t)d_vol % 16 == 0);
     hipLaunchKernelGGL(normalise_kernel, grid, dim3(256), 0, s, d_vol, w, h, nz, d_mm, vec4,
-                       d_row, d_col);
+                       d_row, d_col, d_rflag, d_cflag);
     return hipGetLastError();
 }
 
@@ -782,9 +847,9 @@ size_t radon_side_floats(int w, int bins, int nz, int na) {
 // Grid of the 16-slice walk over n angles on nx XCDs: units are pairs of
 // z-blocks, ~150 of its 1024-thread workgroups per XCD and round (40-600
 // measured alike at 512^3); a multiple of nx workgroups (none when n = 0).
-static Walk16 walk16(const float4* rows, int w, int h, int nz, int bins, const float* cos_t,
-                     const float* sin_t, const int* alist, int n, int nx) {
-    Walk16 W{rows, w, h, cos_t, sin_t, alist, n, 1, 0u};
+static Walk16 walk16(const float4* rows, const int* flag, int w, int h, int nz, int bins,
+                     const float* cos_t, const float* sin_t, const int* alist, int n, int nx) {
+    Walk16 W{rows, flag, w, h, cos_t, sin_t, alist, n, 1, 0u};
     if (n <= 0) return W;
     const int nzb = (nz + kRZ - 1) / kRZ, bb = (bins + 511) / 512;
     const int npair = (nzb + 1) / 2, nap = (n + 1) / 2;
@@ -801,7 +866,8 @@ static void launch_seams(float* d_sino, const float* d_side, const int* d_alist,
                        d_sino, d_side, d_alist, n, na, bins, NT, nseam);
 }
 
-hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, int bins,
+hipError_t launch_radon_rows(const float4* d_img, const int* d_flag, int w, int h, int nz,
+                             int na, int bins,
                              const float* d_cos, const float* d_sin, const int* d_alist, int n,
                              float* d_sino, float* d_side, hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -817,9 +883,9 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
     const long long slots = (bb * n + gsz - 1) / gsz, P = kXcds / gsz;
     const dim3 grid((unsigned)(kXcds * slots * ((nzb + P - 1) / P)));
     if (NT == 512) {
-        const Walk16 W = walk16(d_img, w, h, nz, bins, d_cos, d_sin, d_alist, n, kXcds);
-        hipLaunchKernelGGL(radon_rows16_kernel, dim3(W.blocks), dim3(1024), 0, s, d_img, w, h, nz,
-                           na, bins, d_cos, d_sin, d_alist, d_sino, d_side, n, W.gsz);
+        const Walk16 W = walk16(d_img, d_flag, w, h, nz, bins, d_cos, d_sin, d_alist, n, kXcds);
+        hipLaunchKernelGGL(radon_rows16_kernel, dim3(W.blocks), dim3(1024), 0, s, d_img, d_flag,
+                           w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side, n, W.gsz);
     } else if (NT == 768)
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 776, 768, true>), grid, dim3(768), 0,
                            s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
@@ -838,13 +904,14 @@ hipError_t launch_radon_rows(const float4* d_img, int w, int h, int nz, int na, 
 
 bool radon_walks_paired(int w, int h) { return radon_lanes(w) == 512 && radon_lanes(h) == 512; }
 
-hipError_t launch_radon_walks(const float4* d_row, const float4* d_col, int w, int h, int nz,
-                              int na, int bins, const float* d_cos, const float* d_sin,
-                              const int* d_alist, int n_row, float* d_sino, float* d_side,
-                              hipStream_t s) {
-    const Walk16 a = walk16(d_row, w, h, nz, bins, d_cos, d_sin, d_alist, n_row, kHalfXcds);
-    const Walk16 b = walk16(d_col, h, w, nz, bins, d_sin, d_cos, d_alist + n_row, na - n_row,
-                            kHalfXcds);
+hipError_t launch_radon_walks(const float4* d_row, const float4* d_col, const int* d_rflag,
+                              const int* d_cflag, int w, int h, int nz, int na, int bins,
+                              const float* d_cos, const float* d_sin, const int* d_alist,
+                              int n_row, float* d_sino, float* d_side, hipStream_t s) {
+    const Walk16 a =
+        walk16(d_row, d_rflag, w, h, nz, bins, d_cos, d_sin, d_alist, n_row, kHalfXcds);
+    const Walk16 b = walk16(d_col, d_cflag, h, w, nz, bins, d_sin, d_cos, d_alist + n_row,
+                            na - n_row, kHalfXcds);
     const unsigned half = std::max(a.blocks, b.blocks);  // multiples of kHalfXcds
     if (half == 0) return hipSuccess;
     hipLaunchKernelGGL(radon_rows16_pair_kernel, dim3(2 * half), dim3(1024), 0, s, a, b, nz, na,

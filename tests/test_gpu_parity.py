@@ -139,6 +139,31 @@ def test_sinogram_first_bin_edge_bit_exact(backend, O, h, w, na, bins):
         assert _bits_equal(got.data, ref), (y, x)
 
 
+def test_sinogram_zero_row_skip_bit_exact(backend, O):
+    """The 16-slice walk skips the rows (over the transposed image, the
+    columns) whose 16 normalised slices are all zero. Slices of 512 x 512
+    with one or two nonzero pixels: rows marked in only one z-block of a
+    pair, the last pair, whose second z-block stands in for a missing one (24
+    slices: 3 z-blocks), a pixel that normalises to exactly 0 (the slice's
+    minimum, for which Serial adds +0), and an empty slice. Every bin takes
+    at most one nonzero deposit: bit for bit against Serial."""
+    n, d, na = 512, 24, 90
+    vol = np.zeros((d, n, n), np.float32)
+    vol[2, 5, 100] = 1.7       # z-block 0 of pair 0 only
+    vol[13, 300, 7] = 0.9      # z-block 1 of pair 0 only
+    vol[8, 50, 60] = -1.0      # the slice minimum: normalises to 0, its row stays unmarked
+    vol[8, 400, 300] = 2.0
+    vol[17, 511, 511] = 3.0    # z-block 2, pair 1 (its second z-block stands in)
+    vol[23, 0, 0] = 0.25
+    # slice 20 stays empty
+    _, sino = backend.reconstructVolume(vol, ct.ReconstructionParams(ct.FilterType.Ramp, na),
+                                        want_sinograms=True)
+    for z in range(d):
+        ref, _ = O.compute_sinogram(vol[z], na, n)
+        assert _bits_equal(sino[z], ref), z
+    assert np.all(sino[20] == 0.0)
+
+
 def test_empty_inputs(backend):
     s = backend.computeSinogram(np.zeros((0, 0), np.float32), 10, 10)
     assert s.data.size == 0
