@@ -55,8 +55,8 @@ FLOP_PER_SAMPLE_DEPOSIT = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows16_pair_kernel": 8.007e+10,
-    "bp_kernel": 5.629e+09,
+    "radon_rows16_pair_kernel": 1.693e+10,
+    "bp_kernel": 5.818e+09,
 }
 
 
@@ -583,6 +583,36 @@ def n_row_angles(na: int) -> int:
     return int((np.abs(np.sin(th)) <= np.abs(np.cos(th))).sum())
 
 
+def dense_input(ctx, vol_host, n, na, ftype, updates):
+    """The same step on a volume with no zero pixel (the phantom plus
+    1 + 0.001 x, so every slice keeps a span): the Radon walks skip no row,
+    which is their cost on a dense volume."""
+    nvox = n * n * n
+    dense = vol_host + (np.float32(1.0) + np.float32(1e-3) * np.arange(n, dtype=np.float32))
+    d_dense = ctx.alloc(4 * nvox).upload(dense)
+    d_out = ctx.alloc(4 * nvox)
+    dense_walk = walked_deposits(dense, na, n_row_angles(na))
+    del dense
+    ctx.reconstruct_volume_dev(d_dense, n, n, n, na, ftype, d_out)
+    ctx.synchronize()
+    kd = {}
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ctx.reconstruct_volume_dev(d_dense, n, n, n, na, ftype, d_out)
+        for key, v in ctx.kernel_timings().items():
+            kd[key] = kd.get(key, 0.0) + v / 3
+    ctx.synchronize()
+    ms_dense = (time.perf_counter() - t0) * 1e3 / 3
+    d_dense.free()
+    d_out.free()
+    return {"ms_per_step": ms_dense, "voxel_updates_per_s": updates / (ms_dense / 1e3),
+            "radon_walks_ms": kd["walk_image"] + kd["walk_transpose"], "kernels_ms": kd,
+            "rows_walked_frac": dense_walk["rows_walked_frac"],
+            "columns_walked_frac": dense_walk["columns_walked_frac"],
+            "note": "the same step on the phantom plus 1 + 0.001 x (no zero pixel): no row of "
+                    "the Radon walks is skipped"}
+
+
 def api_form(vol_host, n, na, ftype, reps=10, warm_s=0.6):
     """reconstructVolume on host buffers (ct_hip_reconstruct_volume): the
     volume in and out through pinned staging, z-slabs pipelined so the PCIe
@@ -643,6 +673,8 @@ def main():
                     help="CPU work per baseline run (the sample grows to this; 3 runs)")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-api", action="store_true")
+    ap.add_argument("--no-dense", action="store_true",
+                    help="skip the step on a volume with no zero pixel (dense_input)")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--forms", default="metrics,precision,batched,served",
                     help="extra forms to measure: metrics,precision,batched,served (or none)")
@@ -718,32 +750,10 @@ def main():
     log(f"[bench] roofline {roofline}")
 
     extra = {}
-    # The same step on a volume with no zero pixel (the phantom plus
-    # 1 + 0.001 x, so every slice keeps a span): the Radon walks skip no row,
-    # which is their cost on a dense volume.
-    dense = vol_host + (np.float32(1.0) + np.float32(1e-3) * np.arange(n, dtype=np.float32))
-    d_dense = ctx.alloc(4 * nvox).upload(dense)
-    dense_walk = walked_deposits(dense, na, n_row_angles(na))
-    del dense
-    ctx.reconstruct_volume_dev(d_dense, n, n, n, na, ftype, d_out)
-    ctx.synchronize()
-    kd = {}
-    t0 = time.perf_counter()
-    for _ in range(3):
-        ctx.reconstruct_volume_dev(d_dense, n, n, n, na, ftype, d_out)
-        for key, v in ctx.kernel_timings().items():
-            kd[key] = kd.get(key, 0.0) + v / 3
-    ctx.synchronize()
-    ms_dense = (time.perf_counter() - t0) * 1e3 / 3
-    d_dense.free()
-    extra["dense_input"] = {
-        "ms_per_step": ms_dense, "voxel_updates_per_s": updates / (ms_dense / 1e3),
-        "radon_walks_ms": kd["walk_image"] + kd["walk_transpose"], "kernels_ms": kd,
-        "rows_walked_frac": dense_walk["rows_walked_frac"],
-        "columns_walked_frac": dense_walk["columns_walked_frac"],
-        "note": "the same step on the phantom plus 1 + 0.001 x (no zero pixel): no row of the "
-                "Radon walks is skipped"}
-    log(f"[bench] dense input: {ms_dense:.2f} ms/step, walks {extra['dense_input']['radon_walks_ms']:.2f} ms")
+    if not args.no_dense:
+        extra["dense_input"] = dense_input(ctx, vol_host, n, na, ftype, updates)
+        log(f"[bench] dense input: {extra['dense_input']['ms_per_step']:.2f} ms/step, "
+            f"walks {extra['dense_input']['radon_walks_ms']:.2f} ms")
     d_sino = None
     if not args.no_parity or "precision" in args.forms:
         d_sino = ctx.alloc(4 * n * na * n)
