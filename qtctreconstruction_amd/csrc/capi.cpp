@@ -329,7 +329,7 @@ size_t nzb_of(int nz) { return (size_t)((nz + kZB - 1) / kZB); }
 struct RadonWs {
     unsigned* keys;
     float4 *row, *col;
-    int* rflag;  // [zb][h] row flags of the row copy, then [zb][w] of the column copy
+    int* rflag;  // [zb][h][2] row extents of the row copy, then [zb][w][2] of the column copy
     float* side;
 };
 
@@ -339,7 +339,7 @@ static RadonWs radon_ws(ct_hip_ctx* c, int w, int h, int nz, int na, int bins) {
     const size_t rzb = (size_t)((nz + kRZ - 1) / kRZ);
     ws.row = ensure<float4>(c, c->row, rzb * radon_zbytes(w, h));
     ws.col = ensure<float4>(c, c->col, rzb * radon_zbytes(h, w));
-    ws.rflag = ensure<int>(c, c->rflag, sizeof(int) * rzb * ((size_t)h + w));
+    ws.rflag = ensure<int>(c, c->rflag, sizeof(int) * 2 * rzb * ((size_t)h + w));
     const size_t nside = std::max(radon_side_floats(w, bins, nz, na),
                                   radon_side_floats(h, bins, nz, na));
     ws.side = nside ? ensure<float>(c, c->side, sizeof(float) * nside) : nullptr;
@@ -358,9 +358,9 @@ void stage_radon(ct_hip_ctx* c, const float* d_vol, int w, int h, int nz, int na
     hip_check(launch_minmax(d_vol, (size_t)w * h, nz, keys, c->stream), "minmax");
     hip_check(launch_minmax_decode(keys, nz, d_mm, c->stream), "minmax decode");
     const int nzb = (nz + kRZ - 1) / kRZ;
-    int* rflag = ws.rflag;                     // rows of the row copy
-    int* cflag = ws.rflag + (size_t)nzb * h;  // rows of the column copy (image columns)
-    hip_check(launch_zero_ints(rflag, nzb * (h + w), c->stream), "zero row flags");
+    int* rflag = ws.rflag;                         // rows of the row copy
+    int* cflag = ws.rflag + (size_t)nzb * h * 2;  // rows of the column copy (image columns)
+    hip_check(launch_zero_ints(rflag, 2 * nzb * (h + w), c->stream), "zero row extents");
     hip_check(hipEventRecord(c->evr[1], c->stream), "event");
     hip_check(launch_normalise(d_vol, w, h, nz, d_mm, row, col, rflag, cflag, c->stream),
               "normalise");

@@ -113,8 +113,9 @@ hipError_t launch_minmax_decode(const unsigned* d_keys, int nz, float* d_mm, hip
 // w = (raw == 0) ? 0 : (raw - min) * (1/span); z-blocked row copy
 // [zb][h][w + kPadX] and column copy (the transpose) [zb][w][h + kPadX],
 // kRZ = 8 slices per pixel (2 float4), kPadX zero pixels past each row.
-// d_rflag [zb][h] and d_cflag [zb][w] (zeroed beforehand) get 1 where the
-// z-block's row (column) holds a nonzero normalised pixel.
+// d_rflag [zb][h][2] and d_cflag [zb][w][2] (zeroed beforehand) get each
+// z-block row's {max(x + 1), max(w - x)} over its nonzero normalised pixels,
+// and each column's {max(y + 1), max(h - y)}: {0, 0} for one of zeros.
 hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const float* d_mm,
                             float4* d_row, float4* d_col, int* d_rflag, int* d_cflag,
                             hipStream_t s);
@@ -123,8 +124,9 @@ hipError_t launch_normalise(const float* d_vol, int w, int h, int nz, const floa
 // Steeper angles run it on the column copy with (w, h) and (d_cos, d_sin)
 // exchanged. d_side holds radon_side_floats(w, bins, nz, na) floats (the
 // sums that cross bin blocks; none when every angle fits one block).
-// d_flag [zb][h]: the copy's row flags (launch_normalise); the 16-slice walk
-// skips the rows that are zero in both of its z-blocks.
+// d_flag [zb][h][2]: the copy's row extents (launch_normalise); the 16-slice
+// walk skips the rows that are zero in both of its z-blocks, and the waves
+// whose candidates lie outside a row's extent.
 size_t radon_side_floats(int w, int bins, int nz, int na);
 hipError_t launch_radon_rows(const float4* d_img, const int* d_flag, int w, int h, int nz,
                              int na, int bins, const float* d_cos, const float* d_sin,

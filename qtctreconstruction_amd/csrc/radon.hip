@@ -133,10 +133,13 @@ __global__ void minmax_decode_kernel(const unsigned* keys, int nz, float* mm) {
 // on consecutive 16-byte units: 1 KB contiguous per wave store in the row
 // copy, a column's 512 bytes per 32 lanes in the column copy. Tile rows are padded by one
 // float4, so the column reads (16 rows apart) spread over the banks.
-// rflag[zb][y] and cflag[zb][x] (zero beforehand) are set to 1 where row y
-// (column x) of the z-block holds a nonzero normalised pixel: a row of zeros
-// adds nothing to any bin (SerialBackend.cpp:104 skips zero pixels), and the
-// 16-slice walk skips it.
+// rflag[zb][y] = {max(x + 1), max(w - x)} over the nonzero normalised pixels
+// (x, y) of the z-block, and cflag[zb][x] = {max(y + 1), max(h - y)} over
+// those of column x (both zero beforehand; {0, 0} for a row or column of
+// zeros): the nonzero extent of each row and column. Zero pixels add nothing
+// to any bin (SerialBackend.cpp:104 skips them): the 16-slice walk skips the
+// rows of zeros, and the waves whose candidates all lie outside a row's
+// extent.
 constexpr int kNormTW = 64, kNormTH = 16;
 constexpr int kNormRow = kNormTW * 2 + 1;  // float4 per padded tile row
 __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict__ vol, int w,
@@ -147,9 +150,9 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
                                                         int* __restrict__ rflag,
                                                         int* __restrict__ cflag) {
     __shared__ float4 tile[kNormTH * kNormRow];
-    __shared__ int rowf[kNormTH], colf[kNormTW];
-    if (threadIdx.x < kNormTH) rowf[threadIdx.x] = 0;
-    if (threadIdx.x < kNormTW) colf[threadIdx.x] = 0;
+    __shared__ int rowe[kNormTH][2], cole[kNormTW][2];
+    if (threadIdx.x < 2 * kNormTH) (&rowe[0][0])[threadIdx.x] = 0;
+    if (threadIdx.x < 2 * kNormTW) (&cole[0][0])[threadIdx.x] = 0;
     __syncthreads();
     const int zb = blockIdx.z;
     const int bx = blockIdx.x * kNormTW, by = blockIdx.y * kNormTH;
@@ -204,16 +207,18 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
                 const float raw = v[k][j];
                 v[k][j] = (raw == 0.0f) ? 0.0f : (raw - mn[k]) * inv[k];
             }
-        bool any = false;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            bool col = false;
+            bool nz = false;
 #pragma unroll
-            for (int k = 0; k < kRZ; ++k) col |= v[k][j] != 0.0f;  // NaN counts
-            if (col) colf[4 * tx + j] = 1;
-            any |= col;
+            for (int k = 0; k < kRZ; ++k) nz |= v[k][j] != 0.0f;  // NaN counts
+            if (nz) {  // a nonzero pixel lies inside the image: x + j < w, y < h
+                atomicMax(&rowe[ty][0], x + j + 1);
+                atomicMax(&rowe[ty][1], w - (x + j));
+                atomicMax(&cole[4 * tx + j][0], y + 1);
+                atomicMax(&cole[4 * tx + j][1], h - y);
+            }
         }
-        if (any) rowf[ty] = 1;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             float4* d = tile + ty * kNormRow + (4 * tx + j) * 2;
@@ -245,10 +250,16 @@ __global__ __launch_bounds__(256) void normalise_kernel(const float* __restrict_
     }
     if (t < kNormTH) {
         const int y = by + t;
-        if (y < h && rowf[t]) rflag[(size_t)zb * h + y] = 1;
+        if (y < h && rowe[t][0] > 0) {
+            atomicMax(&rflag[((size_t)zb * h + y) * 2], rowe[t][0]);
+            atomicMax(&rflag[((size_t)zb * h + y) * 2 + 1], rowe[t][1]);
+        }
     } else if (t < kNormTH + kNormTW) {
         const int c = t - kNormTH, x = bx + c;
-        if (x < w && colf[c]) cflag[(size_t)zb * w + x] = 1;
+        if (x < w && cole[c][0] > 0) {
+            atomicMax(&cflag[((size_t)zb * w + x) * 2], cole[c][0]);
+            atomicMax(&cflag[((size_t)zb * w + x) * 2 + 1], cole[c][1]);
+        }
     }
 }
 
@@ -557,10 +568,12 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
 // u = s - i in [0, 1) exactly when floor(s) = i (one unsigned compare of u's
 // bits); A = fl(1 - u), or 1 on the last bin (SerialBackend.cpp:117-118);
 // B = u (the last bin's B goes nowhere and is not written).
-// Rows whose 16 slices are all zero add +0 to every sum and are skipped: the
-// block first lists, in LDS, the rows that its two z-blocks' row flags mark
-// (launch_normalise), then stages and sums those only, in ascending order, so
-// every sum is the one the full walk forms.
+// Zero pixels add +0 to every sum: the block first lists, in LDS and in
+// ascending order, the rows of its two z-blocks that hold a nonzero pixel,
+// with their joint nonzero extent [xlo, xhi] (launch_normalise), and stages
+// and walks those only; in a row, a wave none of whose lanes' two candidates
+// lies in the extent skips its sums. Every sum is the one the full walk
+// forms.
 constexpr int kR16W = 576;
 
 // The walk of workgroup u of its grid on nx XCDs (the body of both kernels
@@ -577,8 +590,8 @@ __device__ __forceinline__ void rows16_walk(
     // wait for the DMA into the other); [slice group (4 slices)][pixel]
     __shared__ float4 tE[4][kR16W];
     __shared__ float4 tO[4][kR16W];
-    __shared__ unsigned short rows[kRadonMaxEdge];  // the rows to walk
-    __shared__ int nrows;
+    // the rows to walk: y | xlo << 11 | xhi << 21 (y < 2048, x < 1024)
+    __shared__ unsigned rows[kRadonMaxEdge];
     const int bb = (bins + NT - 1) / NT;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int npair = (nzb + 1) / 2;
@@ -626,26 +639,31 @@ __device__ __forceinline__ void rows16_walk(
                                              16, 0, 0);
         }
     };
-    // the rows marked in either z-block (one wave, 64 rows a ballot)
-    if (wave == 0) {
-        const int* f0 = flag + (size_t)(2 * zp) * h;
-        const int* f1 = flag + (size_t)min(2 * zp + 1, nzb - 1) * h;
-        int n = 0;
+    // the rows with a nonzero pixel in either z-block, 64 rows a ballot;
+    // every wave counts them, wave 0 writes the list
+    int nr = 0;
+    {
+        const int* f0 = flag + (size_t)(2 * zp) * h * 2;
+        const int* f1 = flag + (size_t)min(2 * zp + 1, nzb - 1) * h * 2;
         for (int y0 = 0; y0 < h; y0 += 64) {
             const int y = y0 + lane;
-            const bool mark = y < h && (f0[y] | f1[y]) != 0;
+            const int hi1 = y < h ? max(f0[2 * y], f1[2 * y]) : 0;
+            const int lo1 = y < h ? max(f0[2 * y + 1], f1[2 * y + 1]) : 0;
+            const bool mark = hi1 > 0;
             const unsigned long long m = __builtin_amdgcn_ballot_w64(mark);
-            if (mark) rows[n + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (unsigned short)y;
-            n += __builtin_popcountll(m);
+            if (wave == 0 && mark)
+                rows[nr + __builtin_popcountll(m & ((1ull << lane) - 1ull))] =
+                    (unsigned)y | (unsigned)(w - lo1) << 11 | (unsigned)(hi1 - 1) << 21;
+            nr += __builtin_popcountll(m);
         }
-        if (lane == 0) nrows = n;
     }
     __syncthreads();
-    const int nr = __builtin_amdgcn_readfirstlane(nrows);
     float accA[16], accB[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) accA[k] = accB[k] = 0.0f;
-    auto row = [&](const float4 (&t)[4][kR16W], int y) {
+    auto row = [&](const float4 (&t)[4][kR16W], unsigned e) {
+        const int y = (int)(e & 2047u);
+        const int xlo = (int)((e >> 11) & 1023u), xhi = (int)(e >> 21);
         const float yf = (float)y;
         const float sy = (yf - g.cy) * g.sn;  // SerialBackend.cpp:110
         const float x0f = row_first(g, yf);
@@ -677,16 +695,22 @@ __device__ __forceinline__ void rows16_walk(
                 accB[z] = fmaf(p[z], wb, accB[z]);
             }
         };
-        add(0, u0, in0);
-        if (two) add(1, u1, in1);
+        // a wave whose candidates d, d + 1 all lie outside [xlo, xhi] would add
+        // +0 to every sum
+        if (__builtin_amdgcn_ballot_w64(d <= xhi && d + 1 >= xlo) != 0) {
+            add(0, u0, in0);
+            if (two) add(1, u1, in1);
+        }
     };
-    // the list 64 entries at a time, entry k0 + l in lane l; a row is read
+    // the list 64 entries at a time, entry k0 + l in lane l; an entry is read
     // out with v_readlane (a wave-uniform value, no LDS wait per row)
     int k0 = 0;
     int chunk = lane < nr ? (int)rows[lane] : 0;
-    auto listed = [&](int k) { return k < nr ? __builtin_amdgcn_readlane(chunk, k - k0) : 0; };
-    int y0 = listed(0), y1 = listed(1);
-    if (nr > 0) stage(tE, y0);
+    auto listed = [&](int k) {
+        return k < nr ? (unsigned)__builtin_amdgcn_readlane(chunk, k - k0) : 0u;
+    };
+    unsigned y0 = listed(0), y1 = listed(1);
+    if (nr > 0) stage(tE, (int)(y0 & 2047u));
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     for (int k = 0; k < nr; k += 2) {
@@ -696,13 +720,13 @@ __device__ __forceinline__ void rows16_walk(
             k0 += 64;
             chunk = k0 + lane < nr ? (int)rows[k0 + lane] : 0;
         }
-        const int y2 = listed(k + 2), y3 = listed(k + 3);
-        if (k + 1 < nr) stage(tO, y1);
+        const unsigned y2 = listed(k + 2), y3 = listed(k + 3);
+        if (k + 1 < nr) stage(tO, (int)(y1 & 2047u));
         row(tE, y0);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         if (k + 1 < nr) {
-            if (k + 2 < nr) stage(tE, y2);
+            if (k + 2 < nr) stage(tE, (int)(y2 & 2047u));
             row(tO, y1);
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
