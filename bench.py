@@ -12,9 +12,10 @@ voxel-updates/s = 512^3 * 720 / step time, the nominal count of the workload
 only the voxels inside the reconstruction circle, 78.5% of them, the ones
 Serial computes (SerialBackend.cpp:219-223). The Radon walks skip the rows
 whose 16 normalised slices are all zero (Serial skips zero pixels,
-SerialBackend.cpp:104); their roofline counts the deposits of the rows
-walked (`radon_rows_walked`), and `dense_input` gives the same step on a
-volume with no zero pixel.
+SerialBackend.cpp:104), and in a row walked a wave whose candidates all lie
+outside the row's nonzero extent skips its sums; their roofline counts the
+deposits of the rows walked (`radon_rows_walked`), and `dense_input` gives
+the same step on a volume with no zero pixel.
 
 Prints one JSON line. Timed region: K steps between two device
 synchronisations; per-kernel times come from HIP events on the context's
@@ -55,8 +56,8 @@ FLOP_PER_SAMPLE_DEPOSIT = 4
 # 4 n^3 bytes, phantom_kernel with no ellipsoids writes 4 n^3 bytes); the
 # profile summaries list the raw counts and the command.
 TRAFFIC_BYTES = {
-    "radon_rows16_pair_kernel": 1.693e+10,
-    "bp_kernel": 5.818e+09,
+    "radon_rows16_pair_kernel": 7.775e+09,
+    "bp_kernel": 5.808e+09,
 }
 
 
@@ -211,7 +212,7 @@ def dominant_roofline(n, na, kt, launches, radon_dep):
            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": TRAFFIC_BYTES.get(name),
            "kernel": name, "launches_per_step": nl, "avg_launch_ms": t_ms / nl,
            "note": f"{what}; FP32 VALU work, no MFMA; "
-                   f"{'9 FLOP per pixel-angle-slice deposit of the rows walked (rows whose 16 slices are all zero are skipped)' if name.startswith('radon') else '9 FLOP per voxel-update inside the circle'}"
+                   f"{'9 FLOP per pixel-angle-slice deposit of the rows walked (rows whose 16 slices are all zero are skipped, and waves whose candidates lie outside a row nonzero extent skip their sums)' if name.startswith('radon') else '9 FLOP per voxel-update inside the circle'}"
                    "; 157.3 TF is MI355X's FP32 peak, which counts packed FP32; these kernels "
                    "are built without packed FP32 ops (78.6 TF non-packed issue ceiling)"}
     if name.startswith("radon"):
