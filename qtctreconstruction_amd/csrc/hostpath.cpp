@@ -254,7 +254,9 @@ void release_host_paths(ct_hip_ctx* c) {
 
 size_t volume_ws_per_slice(int w, int h, int na, bool sino_ws) {
     const size_t sq = (size_t)geometry(w).sq;
-    return (radon_zbytes(w, h) + radon_zbytes(h, w)) / kRZ + (size_t)na * sq * sizeof(float) +
+    return (radon_zbytes(w, h) + radon_zbytes(h, w)) / kRZ +
+           2 * sizeof(int) * ((size_t)h + w) / kRZ +  // row and column extents
+           (size_t)na * sq * sizeof(float) +
            (sino_ws ? (size_t)na * w * sizeof(float) : 0);
 }
 
