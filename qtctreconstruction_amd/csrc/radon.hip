@@ -392,7 +392,8 @@ __device__ __forceinline__ RayGeom ray_geom(int i, int bins, int w, int h, float
 // cos > 0, the highest for cos < 0), and every lane's chunk lies within
 // kRowW pixels of it: lanes 255 bins apart are at most 255/|cos| <= 360.7
 // pixels apart, plus a chunk of 2. Rows outside a lane's candidate rows add
-// +0.
+// +0, and so do rows of zeros: the block walks, in ascending order, only
+// the rows that the z-block's row extents (launch_normalise) mark.
 //
 // The other angles walk the transposed image: the column copy
 // [zb][w][h + kPadX] is the row copy of the transpose, and the kernel runs on
@@ -412,7 +413,8 @@ constexpr int kRadonRows = 2;  // rows staged per step
 // rays at any angle.
 template <int RR, int U, int kRowW, int NT, bool FULL>
 __global__ __launch_bounds__(NT) void radon_rows_kernel(
-    const float4* __restrict__ rowL, int w, int h, int nz, int na, int bins,
+    const float4* __restrict__ rowL, const int* __restrict__ flag, int w, int h, int nz,
+    int na, int bins,
     const float* __restrict__ cos_t, const float* __restrict__ sin_t,
     const int* __restrict__ alist, float* __restrict__ sino, float* __restrict__ side,
     int n_ang, int gsz) {
@@ -427,6 +429,7 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
     __shared__ float4 t0[RR][kRowW];  // slices 0-3 of the staged pixels
     __shared__ float4 t1[RR][kRowW];  // slices 4-7
     __shared__ int xw_s[2][RR];       // window starts: staged rows, next rows
+    __shared__ unsigned short rows[kRadonMaxEdge];  // the rows to walk
     const int bb = (bins + NT - 1) / NT;
     const int nzb = (nz + kRZ - 1) / kRZ;
     const int slots = (bb * n_ang + gsz - 1) / gsz;
@@ -449,13 +452,32 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
     // the extreme interval's geometry, held by the first RR lanes only: lane
     // r computes the window start of row ys + r of the next step
     const int ie = c > 0.0f ? b0 : min(b0 + NT - 1, bins - 1);
+    // the rows with a nonzero pixel, 64 rows a ballot; every wave counts
+    // them, wave 0 writes the list
+    const int lane = threadIdx.x & 63;
+    int nr = 0;
+    {
+        const int* f = flag + (size_t)zb * h * 2;
+        for (int y0 = 0; y0 < h; y0 += 64) {
+            const int y = y0 + lane;
+            const bool mark = y < h && f[2 * y] > 0;
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(mark);
+            if (threadIdx.x < 64 && mark)
+                rows[nr + __builtin_popcountll(m & ((1ull << lane) - 1ull))] = (unsigned short)y;
+            nr += __builtin_popcountll(m);
+        }
+    }
+    __syncthreads();
+    // listed row k (clamped to the last one: a row staged past the list is
+    // never read)
+    auto listed = [&](int k) { return nr > 0 ? (int)rows[min(k, nr - 1)] : 0; };
     RayGeom ge;
     if (threadIdx.x < RR) {
         if (FULL) {
             xw_s[0][threadIdx.x] = 0;
         } else {
             ge = ray_geom(ie, bins, w, h, c, sn);
-            xw_s[0][threadIdx.x] = (int)row_first(ge, (float)threadIdx.x);
+            xw_s[0][threadIdx.x] = (int)row_first(ge, (float)listed(threadIdx.x));
         }
     }
     // Staging: the first half of the block stages the step's first row, the
@@ -472,10 +494,10 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
 #pragma unroll
     for (int k = 0; k < kRZ; ++k) accA[k] = accB[k] = 0.0f;
     int buf = 0;
-    for (int ys = 0; ys < h; ys += RR, buf ^= 1) {
+    for (int ks = 0; ks < nr; ks += RR, buf ^= 1) {
         __syncthreads();  // the previous rows are consumed; xw_s[buf] is set
         if (!FULL && threadIdx.x < RR)  // window starts of the next step's rows
-            xw_s[buf ^ 1][threadIdx.x] = (int)row_first(ge, (float)(ys + RR + threadIdx.x));
+            xw_s[buf ^ 1][threadIdx.x] = (int)row_first(ge, (float)listed(ks + RR + threadIdx.x));
         // Lanes read window slots up to x0 + U - 1 <= w + kPadX - 1, inside
         // the zero pad, and rows below h only: slots past the pad or the last
         // row are never read, so their addresses are clamped instead of
@@ -483,7 +505,7 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
         {
             // this half's row (clamped to the last row) and window start,
             // both wave-uniform
-            const unsigned ro = (unsigned)min(ys + srow, h - 1) * (unsigned)wp * 32u;
+            const unsigned ro = (unsigned)listed(ks + srow) * (unsigned)wp * 32u;
             const int xr = FULL ? 0 : __builtin_amdgcn_readfirstlane(xw_s[buf][srow]);
             float4 q0[NE], q1[NE];
 #pragma unroll
@@ -502,8 +524,8 @@ __global__ __launch_bounds__(NT) void radon_rows_kernel(
             }
         }
         __syncthreads();
-        for (int r = 0; r < RR && ys + r < h; ++r) {
-            const float yf = (float)(ys + r);
+        for (int r = 0; r < RR && ks + r < nr; ++r) {
+            const float yf = (float)listed(ks + r);
             const float sy = (yf - g.cy) * g.sn;  // SerialBackend.cpp:110
             const float x0f = row_first(g, yf);
             const int d = FULL ? (int)x0f : (int)x0f - xw_s[buf][r];
@@ -912,15 +934,15 @@ hipError_t launch_radon_rows(const float4* d_img, const int* d_flag, int w, int 
                            w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side, n, W.gsz);
     } else if (NT == 768)
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 776, 768, true>), grid, dim3(768), 0,
-                           s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
+                           s, d_img, d_flag, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
                            n, gsz);
     else if (NT == 1024)
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 1032, 1024, true>), grid,
-                           dim3(1024), 0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist,
+                           dim3(1024), 0, s, d_img, d_flag, w, h, nz, na, bins, d_cos, d_sin, d_alist,
                            d_sino, d_side, n, gsz);
     else
         hipLaunchKernelGGL((radon_rows_kernel<kRadonRows, 2, 368, 256, false>), grid, dim3(256),
-                           0, s, d_img, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
+                           0, s, d_img, d_flag, w, h, nz, na, bins, d_cos, d_sin, d_alist, d_sino, d_side,
                            n, gsz);
     if (bb > 1) launch_seams(d_sino, d_side, d_alist, n, nz, na, bins, NT, s);
     return hipGetLastError();

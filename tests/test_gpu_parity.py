@@ -139,15 +139,18 @@ def test_sinogram_first_bin_edge_bit_exact(backend, O, h, w, na, bins):
         assert _bits_equal(got.data, ref), (y, x)
 
 
-def test_sinogram_zero_row_skip_bit_exact(backend, O):
-    """The 16-slice walk skips the rows (over the transposed image, the
-    columns) whose 16 normalised slices are all zero. Slices of 512 x 512
-    with one or two nonzero pixels: rows marked in only one z-block of a
-    pair, the last pair, whose second z-block stands in for a missing one (24
-    slices: 3 z-blocks), a pixel that normalises to exactly 0 (the slice's
-    minimum, for which Serial adds +0), and an empty slice. Every bin takes
-    at most one nonzero deposit: bit for bit against Serial."""
-    n, d, na = 512, 24, 90
+@pytest.mark.parametrize("n,na", [(512, 90), (600, 24), (1100, 24)])
+def test_sinogram_zero_row_skip_bit_exact(backend, O, n, na):
+    """The walks skip the rows (over the transposed image, the columns) that
+    hold no nonzero normalised pixel: the 16-slice walk (512 px) per pair of
+    z-blocks, with each row's nonzero extent; the 8-slice walk per z-block,
+    whole rows (600 px) or windows (1100 px). Slices with one or two nonzero
+    pixels: rows marked in only one z-block of a pair, the last pair, whose
+    second z-block stands in for a missing one (24 slices: 3 z-blocks), a
+    pixel that normalises to exactly 0 (the slice's minimum, for which
+    Serial adds +0), and an empty slice. Every bin takes at most one nonzero
+    deposit: bit for bit against Serial."""
+    d = 24
     vol = np.zeros((d, n, n), np.float32)
     vol[2, 5, 100] = 1.7       # z-block 0 of pair 0 only
     vol[13, 300, 7] = 0.9      # z-block 1 of pair 0 only
